@@ -1,0 +1,139 @@
+/*
+ * qcu.h -- C ABI of the MI355X (gfx950) multi-RHS SU(N) Wilson dslash and
+ * BiCGStab library (libqcu.so).
+ *
+ * The first eleven entry points keep the names, argument meaning, layouts
+ * and call protocol of the reference library's C ABI (include/qcu.h:7-29 and
+ * src/interface.cu:12-141 of YellowBicycleee/qcu_mrhs), so a caller that
+ * binds the reference can bind this library unchanged.  Every pointer named
+ * "fermion" or "gauge" is DEVICE memory owned by the caller.
+ *
+ * Call protocol (src/interface.cu, src/qcu_interface.cu):
+ *   initGridSize -> loadQcuGauge -> getDslash -> R x pushBackFermions
+ *   -> exactly one of start_dslash / mat_Qcu / qcuInvert (each consumes the
+ *   queue) -> ... -> finalizeQcu
+ *
+ * Errors: the reference aborts through errorQcu().  Here every entry point
+ * records the failure (qcuLastError / qcuLastErrorString), prints it to
+ * stderr and returns; set QCU_ABORT_ON_ERROR=1 in the environment to abort
+ * like the reference.
+ */
+#pragma once
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/qcu.h:7-13 -- index order X, Y, Z, T (src/interface.cu:23-30) */
+typedef struct QcuParam_s {
+    int lattice_size[4];
+} QcuParam;
+
+typedef struct QcuGrid_t {
+    int grid_size[4];
+} QcuGrid;
+
+/* Precision codes.  The reference's QcuPrecision enum lives in its absent
+ * qcu_io submodule; these values follow the declaration order used by every
+ * precision switch of the reference (src/data_format/qcu_data_format_shift.cu:
+ * 82-93).  kQcuPrecisionBFloat16 has no reference counterpart. */
+enum {
+    kQcuPrecisionHalf = 0,
+    kQcuPrecisionSingle = 1,
+    kQcuPrecisionDouble = 2,
+    kQcuPrecisionUndefined = 3,
+    kQcuPrecisionBFloat16 = 4
+};
+
+/* include/qcu_helper_macro.h:7-11 */
+enum { kQcuDslashWilson = 0, kQcuDslashClover = 1 };
+
+/* Error codes returned by qcuLastError(). */
+enum {
+    kQcuSuccess = 0,
+    kQcuErrorNotInitialized = 1,
+    kQcuErrorInvalidValue = 2,
+    kQcuErrorQueue = 3,
+    kQcuErrorUnsupported = 4,
+    kQcuErrorDevice = 5,
+    kQcuErrorNotConverged = 6,
+    kQcuErrorBreakdown = 7
+};
+
+/* ---- reference entry points (include/qcu.h:14-29) -------------------- */
+
+/* src/interface.cu:20-38.  Allocates the internal multi-RHS buffers.
+ * inputFloatPrecision: precision of the user fermion vectors (the
+ * reference's "out_float_precision"); dslashFloatPrecision: precision the
+ * solver iterates in (kQcuPrecisionDouble = plain FP64 BiCGStab; Single /
+ * Half / BFloat16 = reliable-update mixed precision with that inner
+ * precision).  grid_size must be all ones (one GPU). */
+void initGridSize(QcuGrid *grid, QcuParam *param, int n_color, int m_rhs,
+                  int inputFloatPrecision, int dslashFloatPrecision);
+/* src/interface.cu:40-43: queue one (out, in) pair of device pointers. */
+void pushBackFermions(void *fermionOut, void *fermionIn);
+/* src/interface.cu:45-48 / src/qcu_interface.cu:257-268: copy an even/odd
+ * gauge field [mu][parity][t][z][y][x/2][Nc][Nc] into internal storage. */
+void loadQcuGauge(void *gauge, int floatPrecision);
+/* src/interface.cu:50-53 / src/qcu_interface.cu:66-107: kappa = 1/(2(4+mass)). */
+void getDslash(int dslashType, double mass);
+/* src/interface.cu:55-58 / src/qcu_interface.cu:109-172: out = D in on one
+ * parity; queued pointers are half-lattice vectors, `parity` is the output
+ * parity. */
+void start_dslash(int parity, int daggerFlag);
+/* src/interface.cu:59-62 / src/qcu_interface.cu:174-255: out = (1 - kappa D) in
+ * on full-lattice vectors. */
+void mat_Qcu(int daggerFlag);
+/* src/interface.cu:70-73 / src/qcu_interface.cu:280-362: solve M x = b for
+ * every queued (x, b) pair; even/odd preconditioned BiCGStab. */
+void qcuInvert(int max_iteration, double p_max_prec);
+/* src/interface.cu:64-68 */
+void finalizeQcu(void);
+/* src/interface.cu:76-105: lexicographic [mu][t][z][y][x][Nc][Nc] ->
+ * even/odd gauge (device pointers). */
+void gauge_eo_precondition(void *prec_gauge, void *non_prec_gauge, int precision);
+/* src/interface.cu:107-134: inverse of gauge_eo_precondition. */
+void gauge_reverse_eo_precondition(void *non_prec_gauge, void *prec_gauge, int precision);
+/* src/interface.cu:137-141: the reference reads through its absent qcu_io
+ * submodule; this library reads a raw little-endian complex128 file
+ * "<prefix>" holding the lexicographic layout and writes the even/odd field
+ * to the device pointer `gauge`. */
+void read_gauge_from_file(void *gauge, const char *file_path_prefix);
+
+/* ---- additions (no reference counterpart) ----------------------------- */
+
+int qcuLastError(void);
+const char *qcuLastErrorString(void);
+void qcuClearError(void);
+
+typedef struct QcuSolveInfo_s {
+    int converged;             /* 1 if every RHS met the tolerance            */
+    int iterations;            /* BiCGStab iterations (inner iterations)      */
+    int reliable_updates;      /* FP64 true-residual recomputations           */
+    int restarts;              /* breakdown restarts                          */
+    double true_rel_residual;  /* max_r ||b_r - M x_r|| / ||b_r||, FP64, full */
+    double seconds;            /* wall time of the solve (gather..scatter)    */
+} QcuSolveInfo;
+
+/* qcuInvert with status: returns kQcuSuccess or an error code, fills info. */
+int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info);
+
+/* Solver options.  inner_precision: precision of the inner (sloppy) solve,
+ * overriding initGridSize's dslashFloatPrecision; delta: reliable-update
+ * threshold (update when ||r|| < delta * max ||r|| since the last update);
+ * use_preconditioner: 1 = apply the loaded learned right preconditioner. */
+int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditioner);
+
+/* Device-memory plumbing for callers that have no other GPU runtime. */
+void *qcuDeviceMalloc(size_t bytes);
+int qcuDeviceFree(void *ptr);
+int qcuMemcpyHtoD(void *dst, const void *src, size_t bytes);
+int qcuMemcpyDtoH(void *dst, const void *src, size_t bytes);
+int qcuMemsetD(void *dst, int value, size_t bytes);
+int qcuDeviceSynchronize(void);
+
+#ifdef __cplusplus
+}
+#endif

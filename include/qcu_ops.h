@@ -1,0 +1,64 @@
+/*
+ * qcu_ops.h -- stateless device-pointer operations of libqcu.so on the
+ * internal multi-RHS layout.  They expose, as plain C, the internal operator
+ * and BLAS interfaces that the reference keeps in C++:
+ *
+ *   qcuDslashMrhs       <- qcu::Dslash::apply(DslashParam) (include/qcd/qcu_dslash.h:10-74)
+ *   qcuBlas*            <- qcu::qcu_blas functors (include/qcu_blas/qcu_blas_complex_*.h,
+ *                          src/qcu_blas/qcu_complex_{xsay,xpay,axpby,axpbypcz,dotc,norm}.cu)
+ *   qcuColorSpinorGather/Scatter <- colorSpinorGather/Scatter
+ *                          (include/data_format/qcu_data_format_shift.cuh:8-17)
+ *   qcuCopyComplexVector <- copyComplexVector_interface (same header)
+ *
+ * Layouts: MRHS half vector [Vh][4][Nc][R] complex, gauge [4][2][Vh][Nc][Nc]
+ * complex, both in the storage precision `precision` (kQcuPrecision*).
+ * Per-RHS scalar arrays are device arrays of R complex FP64 values (the
+ * reference keeps them in the vector precision).  All calls are asynchronous
+ * on the library stream unless stated; qcuDeviceSynchronize waits.
+ * Return: 0 on success, else a kQcuError* code (see qcu.h).
+ */
+#pragma once
+
+#include "qcu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out (parity `parity`) = D in (parity 1 - parity); optionally
+ * out = xpay + coef * D in when xpay != NULL (else out = coef * D in). */
+int qcuDslashMrhs(void *out, const void *in, const void *gauge, const int lattice[4], int n_color,
+                  int m_rhs, int parity, int dagger, int precision, const void *xpay, double coef);
+
+/* res = x - a_r y  (src/qcu_blas/qcu_complex_xsay.cu) */
+int qcuBlasXsay(void *res, const void *x, const void *a, const void *y, long n, int m_rhs,
+                int precision);
+/* res = x + a_r y  (src/qcu_blas/qcu_complex_xpay.cu) */
+int qcuBlasXpay(void *res, const void *x, const void *a, const void *y, long n, int m_rhs,
+                int precision);
+/* res = a_r x + b_r y  (src/qcu_blas/qcu_complex_axpby.cu) */
+int qcuBlasAxpby(void *res, const void *a, const void *x, const void *b, const void *y, long n,
+                 int m_rhs, int precision);
+/* res = a_r x + b_r y + c_r z  (src/qcu_blas/qcu_complex_axpbypcz.cu) */
+int qcuBlasAxpbypcz(void *res, const void *a, const void *x, const void *b, const void *y,
+                    const void *c, const void *z, long n, int m_rhs, int precision);
+/* res[r] = sum_i conj(a[i*R+r]) b[i*R+r]  -> R complex FP64 (device)
+ * (src/qcu_blas/qcu_complex_dotc.cu) */
+int qcuBlasDotc(void *res, const void *a, const void *b, long n, int m_rhs, int precision);
+/* res[r] = ||a_r||_2 -> R FP64 (device)  (src/qcu_blas/qcu_complex_norm.cu) */
+int qcuBlasNorm(void *res, const void *a, long n, int m_rhs, int precision);
+
+/* Gather R per-RHS vectors (host array of R device pointers) of nsites*4*Nc
+ * complex numbers into [nsites][4][Nc][R], converting precision; scatter is
+ * the inverse. */
+int qcuColorSpinorGather(void *dst, int dst_precision, void *const *src_ptrs, int src_precision,
+                         long nsites, int n_color, int m_rhs);
+int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *src,
+                          int src_precision, long nsites, int n_color, int m_rhs);
+/* dst = src converted between precisions, `count` complex numbers */
+int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_precision,
+                         long count);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,233 @@
+"""Python mirror of the reference's C interface, over libqcu.so.
+
+The functions keep the reference's names and argument meaning
+(include/qcu.h:14-29, src/interface.cu:20-141): ``initGridSize``,
+``pushBackFermions``, ``loadQcuGauge``, ``getDslash``, ``start_dslash``,
+``mat_Qcu``, ``qcuInvert``, ``finalizeQcu``, ``gauge_eo_precondition``,
+``gauge_reverse_eo_precondition``, ``read_gauge_from_file``.  Fermion and
+gauge arguments are :class:`DeviceArray` objects (device memory allocated
+through the library).  Where the reference aborts on misuse, these raise
+:class:`QcuError` (the library records the error code).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import PRECISION, QcuGrid, QcuParam, QcuSolveInfo
+
+__all__ = [
+    "QcuError", "DeviceArray", "PRECISION", "initGridSize", "pushBackFermions", "loadQcuGauge",
+    "getDslash", "start_dslash", "mat_Qcu", "qcuInvert", "qcuInvertEx", "finalizeQcu",
+    "gauge_eo_precondition", "gauge_reverse_eo_precondition", "read_gauge_from_file",
+    "set_solver_options", "dslash_mrhs", "to_storage", "from_storage", "synchronize",
+]
+
+
+class QcuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qcu error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    return _lib.load()
+
+
+def check(code=None):
+    """Raise QcuError if the library recorded an error (and clear it)."""
+    L = lib()
+    err = L.qcuLastError() if code is None else code
+    if err:
+        msg = L.qcuLastErrorString().decode()
+        L.qcuClearError()
+        raise QcuError(err, msg)
+
+
+def synchronize():
+    check(lib().qcuDeviceSynchronize())
+
+
+# ---------------------------------------------------------------- storage
+def _bf16_bits(x: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32)
+    rounding = ((u >> 16) & 1) + 0x7FFF
+    return ((u + rounding) >> 16).astype(np.uint16)
+
+
+def _bf16_to_f32(bits: np.ndarray) -> np.ndarray:
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+def to_storage(a: np.ndarray, prec: int) -> np.ndarray:
+    """complex numpy array -> raw storage in precision ``prec`` (interleaved re/im)."""
+    a = np.ascontiguousarray(a)
+    if prec == PRECISION["double"]:
+        return a.astype(np.complex128)
+    if prec == PRECISION["single"]:
+        return a.astype(np.complex64)
+    f = a.astype(np.complex64).view(np.float32)
+    if prec == PRECISION["half"]:
+        return f.astype(np.float16)
+    if prec == PRECISION["bf16"]:
+        return _bf16_bits(f)
+    raise ValueError(prec)
+
+
+def from_storage(raw: np.ndarray, prec: int) -> np.ndarray:
+    """raw storage -> complex128."""
+    if prec in (PRECISION["double"], PRECISION["single"]):
+        return raw.astype(np.complex128)
+    f = raw.astype(np.float32) if prec == PRECISION["half"] else _bf16_to_f32(raw)
+    return f.view(np.complex64).astype(np.complex128)
+
+
+def storage_bytes(count: int, prec: int) -> int:
+    return count * {0: 4, 1: 8, 2: 16, 4: 4}[prec]
+
+
+class DeviceArray:
+    """A device buffer from the library's allocator."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().qcuDeviceMalloc(max(self.nbytes, 1))
+        if not self.ptr:
+            check()
+            raise QcuError(5, "allocation failed")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    @classmethod
+    def from_complex(cls, a: np.ndarray, prec: int = 2) -> "DeviceArray":
+        return cls.from_numpy(to_storage(a, prec))
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().qcuMemcpyHtoD(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self, dtype, count=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        check(lib().qcuMemcpyDtoH(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def complex(self, count: int, prec: int = 2) -> np.ndarray:
+        dt = {0: np.float16, 1: np.complex64, 2: np.complex128, 4: np.uint16}[prec]
+        n = count * (2 if prec in (0, 4) else 1)
+        return from_storage(self.download(dt, n), prec)
+
+    def zero(self) -> None:
+        check(lib().qcuMemsetD(self.ptr, 0, self.nbytes))
+
+    def offset(self, nbytes: int) -> int:
+        return self.ptr + int(nbytes)
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().qcuDeviceFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(x):
+    return x.ptr if isinstance(x, DeviceArray) else x
+
+
+# ------------------------------------------------------ reference entry points
+def initGridSize(lattice, n_color, m_rhs, inputFloatPrecision=2, dslashFloatPrecision=2,
+                 grid=(1, 1, 1, 1)):
+    g = QcuGrid((ctypes.c_int * 4)(*grid))
+    p = QcuParam((ctypes.c_int * 4)(*lattice))
+    lib().initGridSize(ctypes.byref(g), ctypes.byref(p), int(n_color), int(m_rhs),
+                       int(inputFloatPrecision), int(dslashFloatPrecision))
+    check()
+
+
+def pushBackFermions(fermionOut, fermionIn):
+    lib().pushBackFermions(_p(fermionOut), _p(fermionIn))
+    check()
+
+
+def loadQcuGauge(gauge, floatPrecision=2):
+    lib().loadQcuGauge(_p(gauge), int(floatPrecision))
+    check()
+
+
+def getDslash(dslashType, mass):
+    lib().getDslash(int(dslashType), float(mass))
+    check()
+
+
+def start_dslash(parity, daggerFlag=0):
+    lib().start_dslash(int(parity), int(daggerFlag))
+    check()
+
+
+def mat_Qcu(daggerFlag=0):
+    lib().mat_Qcu(int(daggerFlag))
+    check()
+
+
+def qcuInvert(max_iteration, p_max_prec):
+    lib().qcuInvert(int(max_iteration), float(p_max_prec))
+    check()
+
+
+def qcuInvertEx(max_iteration, tol, raise_on_nonconvergence=False):
+    info = QcuSolveInfo()
+    code = lib().qcuInvertEx(int(max_iteration), float(tol), ctypes.byref(info))
+    res = {k: getattr(info, k) for k, _ in QcuSolveInfo._fields_}
+    res["code"] = code
+    if code and (code != 6 or raise_on_nonconvergence):
+        check(code)
+    if code == 6:
+        lib().qcuClearError()
+    return res
+
+
+def set_solver_options(inner_precision=2, delta=0.1, use_preconditioner=False):
+    check(lib().qcuSetSolverOptions(int(inner_precision), float(delta), int(bool(use_preconditioner))))
+
+
+def finalizeQcu():
+    lib().finalizeQcu()
+
+
+def gauge_eo_precondition(prec_gauge, non_prec_gauge, precision=2):
+    lib().gauge_eo_precondition(_p(prec_gauge), _p(non_prec_gauge), int(precision))
+    check()
+
+
+def gauge_reverse_eo_precondition(non_prec_gauge, prec_gauge, precision=2):
+    lib().gauge_reverse_eo_precondition(_p(non_prec_gauge), _p(prec_gauge), int(precision))
+    check()
+
+
+def read_gauge_from_file(gauge, path):
+    lib().read_gauge_from_file(_p(gauge), str(path).encode())
+    check()
+
+
+# --------------------------------------------------------- stateless ops
+def dslash_mrhs(out, inp, gauge, lattice, n_color, m_rhs, parity, dagger=0, precision=2,
+                xpay=None, coef=1.0):
+    L4 = (ctypes.c_int * 4)(*lattice)
+    check(lib().qcuDslashMrhs(_p(out), _p(inp), _p(gauge), L4, int(n_color), int(m_rhs),
+                              int(parity), int(dagger), int(precision),
+                              _p(xpay) if xpay is not None else None, float(coef)))

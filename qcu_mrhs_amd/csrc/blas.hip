@@ -1,0 +1,466 @@
+// blas.hip -- multi-RHS BLAS, single-pass deterministic reductions and the
+// fused BiCGStab steps.
+//
+// Thread mapping: each workgroup keeps (256/R)*R active lanes, so a lane's
+// RHS index is fixed for its whole grid-stride loop and consecutive lanes
+// touch consecutive complex numbers of the [n][R] layout (coalesced).
+//
+// Reductions are single pass: each block reduces its lanes per RHS through
+// LDS in a fixed order and writes R partials; the last block to finish
+// (device-scope counter, release/acquire fences) sums the partials in block
+// order and runs the op's finisher (e.g. alpha = rho / <r0, v>) on device,
+// so no host round trip is needed.  Results are bit-reproducible for a given
+// (n, R).  Accumulation is FP64 for every storage precision.
+#include "blas.h"
+
+namespace qcu {
+namespace {
+
+template <int P> struct PrecT;
+template <> struct PrecT<kDouble> { using S = double; using T = double; };
+template <> struct PrecT<kSingle> { using S = float; using T = float; };
+template <> struct PrecT<kHalf> { using S = __half; using T = float; };
+template <> struct PrecT<kBF16> { using S = __hip_bfloat16; using T = float; };
+
+#define QCU_PREC_DISPATCH(prec, ...)                                \
+    switch (prec) {                                                 \
+        case kDouble: { constexpr int P = kDouble; __VA_ARGS__; }   \
+        case kSingle: { constexpr int P = kSingle; __VA_ARGS__; }   \
+        case kHalf: { constexpr int P = kHalf; __VA_ARGS__; }       \
+        case kBF16: { constexpr int P = kBF16; __VA_ARGS__; }       \
+        default: return hipErrorInvalidValue;                       \
+    }
+
+__device__ __forceinline__ bool rhs_lane(int R, long &g, long &stride, int &rhs) {
+    const int per_block = (int(blockDim.x) / R) * R;
+    if (int(threadIdx.x) >= per_block) return false;
+    g = long(blockIdx.x) * per_block + threadIdx.x;
+    stride = long(gridDim.x) * per_block;
+    rhs = int(threadIdx.x) % R;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> tc(C64 v) { return {T(v.re), T(v.im)}; }
+
+// ------------------------------------------------------------------ axpy
+template <int KIND, typename CS, typename T>
+__global__ __launch_bounds__(256) void k_axpy(CS *__restrict__ res, const CS *__restrict__ x,
+                                             const CS *__restrict__ y, const CS *__restrict__ z,
+                                             const C64 *__restrict__ a, const C64 *__restrict__ b,
+                                             const C64 *__restrict__ c, long N, int R) {
+    long g, stride;
+    int rhs;
+    if (!rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> ar = tc<T>(a[rhs]);
+    Cx<T> br = {T(0), T(0)}, crr = {T(0), T(0)};
+    if constexpr (KIND == kAxpby || KIND == kAxpbypcz) br = tc<T>(b[rhs]);
+    if constexpr (KIND == kAxpbypcz) crr = tc<T>(c[rhs]);
+    for (long e = g; e < N; e += stride) {
+        const Cx<T> xv = load_cx<T>(x + e);
+        const Cx<T> yv = load_cx<T>(y + e);
+        Cx<T> o;
+        if constexpr (KIND == kXsay) o = xv - ar * yv;
+        else if constexpr (KIND == kXpay) o = xv + ar * yv;
+        else if constexpr (KIND == kAxpby) o = ar * xv + br * yv;
+        else o = ar * xv + br * yv + crr * load_cx<T>(z + e);
+        store_cx(res + e, o);
+    }
+}
+
+// ------------------------------------------------------------- reduction
+template <class Op>
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
+                                                          double *__restrict__ partials,
+                                                          unsigned int *__restrict__ counter) {
+    constexpr int NA = Op::NA;
+    __shared__ double buf[NA][kReduceThreads];
+    __shared__ bool am_last;
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    long g, stride;
+    int rhs;
+    const bool active = rhs_lane(R, g, stride, rhs);
+    if (active) {
+        auto loc = op.local(rhs);
+        for (long e = g; e < N; e += stride) op.elem(e, loc, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) buf[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    const int per_block = (kReduceThreads / R) * R;
+    const int nch = per_block / R;
+    if (int(threadIdx.x) < R) {
+        double s[NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) s[k] = 0.0;
+        for (int j = 0; j < nch; ++j)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) s[k] += buf[k][j * R + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) partials[(long(blockIdx.x) * R + threadIdx.x) * 4 + k] = s[k];
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int prev = atomicAdd(counter, 1u);
+        am_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    // last block: sum partials in block order
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    if (t < per_block) {
+        const int r = t % R, ch = t / R;
+        for (int bb = ch; bb < int(gridDim.x); bb += nch)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) acc[k] += partials[(long(bb) * R + r) * 4 + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NA; ++k) buf[k][t] = acc[k];
+    __syncthreads();
+    if (t < R) {
+        double tot[NA];
+#pragma unroll
+        for (int k = 0; k < NA; ++k) tot[k] = 0.0;
+        for (int j = 0; j < nch; ++j)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) tot[k] += buf[k][j * R + t];
+        op.finish(t, tot);
+    }
+    if (t == 0) *counter = 0u;
+}
+
+template <typename CS>
+struct DotOp {
+    static constexpr int NA = 2;
+    const CS *a, *b;
+    C64 *res;
+    struct L {};
+    __device__ L local(int) const { return {}; }
+    __device__ void elem(long e, L, double (&acc)[NA]) const {
+        const Cx<double> u = load_cx<double>(a + e), v = load_cx<double>(b + e);
+        acc[0] = fma(u.re, v.re, fma(u.im, v.im, acc[0]));
+        acc[1] = fma(u.re, v.im, fma(-u.im, v.re, acc[1]));
+    }
+    __device__ void finish(int r, const double (&t)[NA]) const { res[r] = {t[0], t[1]}; }
+};
+
+template <typename CS>
+struct NormOp {
+    static constexpr int NA = 1;
+    const CS *a;
+    double *res;
+    bool take_sqrt;
+    struct L {};
+    __device__ L local(int) const { return {}; }
+    __device__ void elem(long e, L, double (&acc)[NA]) const {
+        const Cx<double> u = load_cx<double>(a + e);
+        acc[0] = fma(u.re, u.re, fma(u.im, u.im, acc[0]));
+    }
+    __device__ void finish(int r, const double (&t)[NA]) const {
+        res[r] = take_sqrt ? sqrt(t[0]) : t[0];
+    }
+};
+
+template <typename CS>
+struct AlphaOp {  // <r0, v>, alpha = rho / <r0, v>
+    static constexpr int NA = 2;
+    const CS *r0, *v;
+    C64 *st;
+    int R;
+    struct L {};
+    __device__ L local(int) const { return {}; }
+    __device__ void elem(long e, L, double (&acc)[NA]) const {
+        const Cx<double> u = load_cx<double>(r0 + e), w = load_cx<double>(v + e);
+        acc[0] = fma(u.re, w.re, fma(u.im, w.im, acc[0]));
+        acc[1] = fma(u.re, w.im, fma(-u.im, w.re, acc[1]));
+    }
+    __device__ void finish(int r, const double (&t)[NA]) const {
+        const C64 r0v = {t[0], t[1]};
+        st[kR0v * R + r] = r0v;
+        // a zero denominator (converged or all-zero RHS) freezes that RHS
+        st[kAlpha * R + r] = (r0v.re == 0.0 && r0v.im == 0.0) ? C64{0.0, 0.0}
+                                                                : cdiv(st[kRho * R + r], r0v);
+    }
+};
+
+template <typename CS>
+struct OmegaOp {  // <t,s>, <t,t>, omega = <t,s>/<t,t>
+    static constexpr int NA = 3;
+    const CS *t, *s;
+    C64 *st;
+    int R;
+    struct L {};
+    __device__ L local(int) const { return {}; }
+    __device__ void elem(long e, L, double (&acc)[NA]) const {
+        const Cx<double> u = load_cx<double>(t + e), w = load_cx<double>(s + e);
+        acc[0] = fma(u.re, w.re, fma(u.im, w.im, acc[0]));
+        acc[1] = fma(u.re, w.im, fma(-u.im, w.re, acc[1]));
+        acc[2] = fma(u.re, u.re, fma(u.im, u.im, acc[2]));
+    }
+    __device__ void finish(int r, const double (&tt)[NA]) const {
+        const C64 ts = {tt[0], tt[1]}, t2 = {tt[2], 0.0};
+        st[kTs * R + r] = ts;
+        st[kTt * R + r] = t2;
+        st[kOmega * R + r] = t2.re == 0.0 ? C64{0.0, 0.0} : cdiv(ts, t2);
+    }
+};
+
+template <typename CS, typename T>
+struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
+    static constexpr int NA = 3;
+    CS *x, *r;
+    const CS *p, *shat, *s, *t, *r0;
+    C64 *st;
+    int R;
+    struct L { Cx<T> al, om; };
+    __device__ L local(int rhs) const { return {tc<T>(st[kAlpha * R + rhs]), tc<T>(st[kOmega * R + rhs])}; }
+    __device__ void elem(long e, L l, double (&acc)[NA]) const {
+        const Cx<T> sv = load_cx<T>(s + e);
+        const Cx<T> shv = shat == s ? sv : load_cx<T>(shat + e);
+        const Cx<T> xv = load_cx<T>(x + e) + l.al * load_cx<T>(p + e) + l.om * shv;
+        store_cx(x + e, xv);
+        const Cx<T> rv = sv - l.om * load_cx<T>(t + e);
+        store_cx(r + e, rv);
+        // reductions use the stored (rounded) residual
+        const Cx<double> rd = load_cx<double>(r + e);
+        const Cx<double> u = load_cx<double>(r0 + e);
+        acc[0] = fma(rd.re, rd.re, fma(rd.im, rd.im, acc[0]));
+        acc[1] = fma(u.re, rd.re, fma(u.im, rd.im, acc[1]));
+        acc[2] = fma(u.re, rd.im, fma(-u.im, rd.re, acc[2]));
+    }
+    __device__ void finish(int rr, const double (&tt)[NA]) const {
+        st[kRnorm2 * R + rr] = {tt[0], 0.0};
+        const C64 rho_new = {tt[1], tt[2]};
+        st[kRhoNew * R + rr] = rho_new;
+        // beta = ((alpha * rho_new) / omega) / rho (src/solver/bicgstab_solve.cu:350-374)
+        const C64 om = st[kOmega * R + rr], rho = st[kRho * R + rr];
+        C64 beta = {0.0, 0.0};
+        if (!(om.re == 0.0 && om.im == 0.0) && !(rho.re == 0.0 && rho.im == 0.0)) {
+            beta = st[kAlpha * R + rr] * rho_new;
+            beta = cdiv(beta, om);
+            beta = cdiv(beta, rho);
+        }
+        st[kBeta * R + rr] = beta;
+        st[kTmpA * R + rr] = st[kRho * R + rr];  // rho of this iteration (reliable updates)
+        st[kRho * R + rr] = rho_new;
+    }
+};
+
+template <typename CS, typename T>
+__global__ __launch_bounds__(256) void k_step_s(CS *__restrict__ sv, const CS *__restrict__ r,
+                                               const CS *__restrict__ v, const C64 *__restrict__ st,
+                                               long N, int R) {
+    long g, stride;
+    int rhs;
+    if (!rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> al = tc<T>(st[kAlpha * R + rhs]);
+    for (long e = g; e < N; e += stride) store_cx(sv + e, load_cx<T>(r + e) - al * load_cx<T>(v + e));
+}
+
+template <typename CS, typename T>
+__global__ __launch_bounds__(256) void k_step_p(CS *__restrict__ p, const CS *__restrict__ r,
+                                               const CS *__restrict__ v, const C64 *__restrict__ st,
+                                               long N, int R) {
+    long g, stride;
+    int rhs;
+    if (!rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> om = tc<T>(st[kOmega * R + rhs]);
+    const Cx<T> be = tc<T>(st[kBeta * R + rhs] * st[kPScale * R + rhs]);
+    for (long e = g; e < N; e += stride) {
+        const Cx<T> q = load_cx<T>(p + e) - om * load_cx<T>(v + e);
+        store_cx(p + e, load_cx<T>(r + e) + be * q);
+    }
+}
+
+template <typename CD, typename CS>
+__global__ __launch_bounds__(256) void k_convert(CD *__restrict__ dst, const CS *__restrict__ src,
+                                                long N, const double *__restrict__ scale, int R) {
+    long g, stride;
+    int rhs;
+    if (!rhs_lane(R, g, stride, rhs)) return;
+    const double sc = scale ? scale[rhs] : 1.0;
+    for (long e = g; e < N; e += stride) {
+        Cx<double> v = load_cx<double>(src + e);
+        if (scale) v = sc * v;
+        store_cx(dst + e, v);
+    }
+}
+
+__global__ void k_fill(C64 *dst, C64 v, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = v;
+}
+
+inline int stream_blocks(long N, int R) {
+    const int per_block = (256 / R) * R;
+    long want = (N + per_block - 1) / per_block;
+    const long cap = 256L * 16;  // 16 workgroups per CU on 256 CUs
+    return int(want < cap ? (want > 0 ? want : 1) : cap);
+}
+
+template <class Op>
+hipError_t run_reduce(const Op &op, long n, int R, ReduceWork &w, int slot, hipStream_t s) {
+    if (R < 1 || R > kReduceThreads) return hipErrorInvalidValue;
+    const int nb = reduce_blocks(n, R, w.max_blocks);
+    hipLaunchKernelGGL(k_reduce<Op>, dim3(nb), dim3(kReduceThreads), 0, s, op, n * R, R,
+                       w.partials, w.counters + slot);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int reduce_blocks(long n, int R, int max_blocks) {
+    const int per_block = (kReduceThreads / R) * R;
+    const long N = n * R;
+    long nb = (N + per_block * 8L - 1) / (per_block * 8L);  // >= 8 entries per lane
+    if (nb > max_blocks) nb = max_blocks;
+    if (nb < 1) nb = 1;
+    return int(nb);
+}
+
+hipError_t reduce_work_alloc(ReduceWork &w, int R) {
+    w.max_blocks = 1024;
+    QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R));
+    QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots));
+    QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots));
+    QCU_HIP_TRY(hipDeviceSynchronize());
+    return hipSuccess;
+}
+
+void reduce_work_free(ReduceWork &w) {
+    if (w.partials) (void)hipFree(w.partials);
+    if (w.counters) (void)hipFree(w.counters);
+    w = ReduceWork{};
+}
+
+hipError_t launch_axpy(const AxpyArgs &a, hipStream_t s) {
+    if (a.R < 1 || a.R > 256) return hipErrorInvalidValue;
+    const long N = a.n * a.R;
+    if (N == 0) return hipSuccess;
+    const int nb = stream_blocks(N, a.R);
+    QCU_PREC_DISPATCH(a.prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        auto res = static_cast<CS *>(a.res);
+        auto x = static_cast<const CS *>(a.x);
+        auto y = static_cast<const CS *>(a.y);
+        auto z = static_cast<const CS *>(a.z);
+        switch (a.kind) {
+            case kXsay: hipLaunchKernelGGL((k_axpy<kXsay, CS, T>), dim3(nb), dim3(256), 0, s, res, x, y, z, a.a, a.b, a.c, N, a.R); break;
+            case kXpay: hipLaunchKernelGGL((k_axpy<kXpay, CS, T>), dim3(nb), dim3(256), 0, s, res, x, y, z, a.a, a.b, a.c, N, a.R); break;
+            case kAxpby: hipLaunchKernelGGL((k_axpy<kAxpby, CS, T>), dim3(nb), dim3(256), 0, s, res, x, y, z, a.a, a.b, a.c, N, a.R); break;
+            case kAxpbypcz: hipLaunchKernelGGL((k_axpy<kAxpbypcz, CS, T>), dim3(nb), dim3(256), 0, s, res, x, y, z, a.a, a.b, a.c, N, a.R); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    })
+}
+
+hipError_t launch_dotc(C64 *res, const void *a, const void *b, long n, int R, int prec,
+                       ReduceWork &w, int slot, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        DotOp<CS> op{static_cast<const CS *>(a), static_cast<const CS *>(b), res};
+        return run_reduce(op, n, R, w, slot, s);
+    })
+}
+
+hipError_t launch_norm(double *res, const void *a, long n, int R, int prec, bool take_sqrt,
+                       ReduceWork &w, int slot, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        NormOp<CS> op{static_cast<const CS *>(a), res, take_sqrt};
+        return run_reduce(op, n, R, w, slot, s);
+    })
+}
+
+hipError_t launch_convert(void *dst, int dst_prec, const void *src, int src_prec, long count,
+                          const double *scale, int R, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    if (R < 1 || R > 256) return hipErrorInvalidValue;
+    const int nb = stream_blocks(count, R);
+    QCU_PREC_DISPATCH(dst_prec, {
+        using CD = typename CxStore<typename PrecT<P>::S>::type;
+        switch (src_prec) {
+            case kDouble: hipLaunchKernelGGL((k_convert<CD, Cx<double>>), dim3(nb), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Cx<double> *>(src), count, scale, R); break;
+            case kSingle: hipLaunchKernelGGL((k_convert<CD, Cx<float>>), dim3(nb), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Cx<float> *>(src), count, scale, R); break;
+            case kHalf: hipLaunchKernelGGL((k_convert<CD, Half2Store>), dim3(nb), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Half2Store *>(src), count, scale, R); break;
+            case kBF16: hipLaunchKernelGGL((k_convert<CD, Bf162Store>), dim3(nb), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Bf162Store *>(src), count, scale, R); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    })
+}
+
+hipError_t launch_step_alpha(C64 *st, const void *r0, const void *v, long n, int R, int prec,
+                             ReduceWork &w, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        AlphaOp<CS> op{static_cast<const CS *>(r0), static_cast<const CS *>(v), st, R};
+        return run_reduce(op, n, R, w, 0, s);
+    })
+}
+
+hipError_t launch_step_s(void *sv, const void *r, const void *v, const C64 *st, long n, int R,
+                         int prec, hipStream_t s) {
+    const long N = n * R;
+    const int nb = stream_blocks(N, R);
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        hipLaunchKernelGGL((k_step_s<CS, T>), dim3(nb), dim3(256), 0, s, static_cast<CS *>(sv),
+                           static_cast<const CS *>(r), static_cast<const CS *>(v), st, N, R);
+        return hipGetLastError();
+    })
+}
+
+hipError_t launch_step_omega(C64 *st, const void *t, const void *sv, long n, int R, int prec,
+                             ReduceWork &w, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        OmegaOp<CS> op{static_cast<const CS *>(t), static_cast<const CS *>(sv), st, R};
+        return run_reduce(op, n, R, w, 1, s);
+    })
+}
+
+hipError_t launch_step_xr(void *x, void *r, const void *p, const void *shat, const void *sv,
+                          const void *t, const void *r0, C64 *st, long n, int R, int prec,
+                          ReduceWork &w, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        XROp<CS, T> op{static_cast<CS *>(x), static_cast<CS *>(r), static_cast<const CS *>(p),
+                       static_cast<const CS *>(shat), static_cast<const CS *>(sv),
+                       static_cast<const CS *>(t),
+                       static_cast<const CS *>(r0), st, R};
+        return run_reduce(op, n, R, w, 2, s);
+    })
+}
+
+hipError_t launch_step_p(void *p, const void *r, const void *v, const C64 *st, long n, int R,
+                         int prec, hipStream_t s) {
+    const long N = n * R;
+    const int nb = stream_blocks(N, R);
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        hipLaunchKernelGGL((k_step_p<CS, T>), dim3(nb), dim3(256), 0, s, static_cast<CS *>(p),
+                           static_cast<const CS *>(r), static_cast<const CS *>(v), st, N, R);
+        return hipGetLastError();
+    })
+}
+
+hipError_t launch_scalar_fill(C64 *dst, C64 value, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(div_up(count, 256)), dim3(256), 0, s, dst, value, count);
+    return hipGetLastError();
+}
+
+}  // namespace qcu

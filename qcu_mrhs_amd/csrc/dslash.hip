@@ -1,0 +1,196 @@
+// dslash.hip -- one-parity multi-RHS SU(N) Wilson hopping term on gfx950.
+//
+// Operator (include/kernel/sun_mrhs_wilson_dslash_simt.cuh:97-181 of the
+// reference; output parity p, input on 1-p):
+//   out(x) = sum_mu (1 - g_mu) U_mu(x) psi(x+mu) + (1 + g_mu) U_mu(x-mu)^+ psi(x-mu)
+// dagger flips the sign of every gamma (ibid. :147,176).  Spin structure via
+// two projected rows per hop: h_r = psi_r + s_r psi_m(r), out_r += U h_r,
+// out_m(r) += c_r U h_r (include/kernel/qcu_gamma.cuh:47-324).
+//
+// VALU kernel: one lane per (site, rhs).  The RHS-innermost layout makes the
+// R lanes of one site read R consecutive complex numbers for every
+// (spin, colour) -- coalesced 16-byte loads; the link is a broadcast load
+// shared by those lanes.  Workgroup = R x (256/R) lanes, blocks remapped so
+// every XCD sweeps a contiguous slab of sites (neighbour reuse in its L2).
+// An optional epilogue fuses out = x + coef * D psi (Schur operator,
+// full-lattice M, source preparation).
+#include "dslash.h"
+
+namespace qcu {
+namespace {
+
+// projection / reconstruction tables, expressed as powers of i for the
+// (1 + gamma) form; the forward hop and the dagger each add a factor -1 = i^2.
+// mat2 (qcu_gamma.cuh:47-51,101-106,159-163,208-212), projection scale
+// (:61-67,115-123,172-180,222-228), reconstruction scale (:52-59,107-114,
+// 164-171,213-220).
+__host__ __device__ constexpr int kMat2(int mu, int r) { return mu < 2 ? 3 - r : 2 + r; }
+__host__ __device__ constexpr int kProjK(int mu, int r) {
+    return mu == 0 ? 1 : mu == 1 ? (r == 0 ? 2 : 0) : mu == 2 ? (r == 0 ? 1 : 3) : 0;
+}
+__host__ __device__ constexpr int kReconK(int mu, int r) {
+    return mu == 0 ? 3 : mu == 1 ? (r == 0 ? 2 : 0) : mu == 2 ? (r == 0 ? 3 : 1) : 0;
+}
+
+template <typename T, typename CS, typename CG, int NC, int MU, bool FWD, bool DAG>
+__device__ __forceinline__ void hop(Cx<T> (&acc)[4][NC], const CS *__restrict__ psi, int R,
+                                    const CG *__restrict__ U) {
+    constexpr int sgn = (FWD != DAG) ? 2 : 0;
+    constexpr int m0 = kMat2(MU, 0), m1 = kMat2(MU, 1);
+    constexpr int kp0 = kProjK(MU, 0) + sgn, kp1 = kProjK(MU, 1) + sgn;
+    constexpr int kr0 = kReconK(MU, 0) + sgn, kr1 = kReconK(MU, 1) + sgn;
+    Cx<T> h0[NC], h1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const Cx<T> a0 = load_cx<T>(psi + (0 * NC + c) * R);
+        const Cx<T> a1 = load_cx<T>(psi + (1 * NC + c) * R);
+        const Cx<T> b0 = load_cx<T>(psi + (m0 * NC + c) * R);
+        const Cx<T> b1 = load_cx<T>(psi + (m1 * NC + c) * R);
+        h0[c] = a0 + ipow<kp0>(b0);
+        h1[c] = a1 + ipow<kp1>(b1);
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        Cx<T> s0 = {T(0), T(0)}, s1 = {T(0), T(0)};
+#pragma unroll
+        for (int b = 0; b < NC; ++b) {
+            if constexpr (FWD) {
+                const Cx<T> u = load_cx<T>(U + a * NC + b);
+                cfma(s0, u, h0[b]);
+                cfma(s1, u, h1[b]);
+            } else {
+                const Cx<T> u = load_cx<T>(U + b * NC + a);   // (U^+)[a][b] = conj(U[b][a])
+                cfma_conj(s0, u, h0[b]);
+                cfma_conj(s1, u, h1[b]);
+            }
+        }
+        acc[0][a] = acc[0][a] + s0;
+        acc[m0][a] = acc[m0][a] + ipow<kr0>(s0);
+        acc[1][a] = acc[1][a] + s1;
+        acc[m1][a] = acc[m1][a] + ipow<kr1>(s1);
+    }
+}
+
+template <typename T, typename CS, typename CG, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_wilson_dslash_valu(CS *__restrict__ out,
+                                                           const CS *__restrict__ in,
+                                                           const CG *__restrict__ gauge,
+                                                           const CS *__restrict__ xpay, T coef,
+                                                           Geom g, int R, int parity) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int site = b * blockDim.y + threadIdx.y;
+    if (site >= g.Vh) return;
+    const int r = threadIdx.x;
+
+    const int Lx = g.L[0], Ly = g.L[1], Lz = g.L[2], Lt = g.L[3], Lxh = g.Lxh;
+    const int xh = site % Lxh;
+    int rest = site / Lxh;
+    const int y = rest % Ly;
+    rest /= Ly;
+    const int z = rest % Lz;
+    const int t = rest / Lz;
+    const int x = 2 * xh + ((y + z + t + parity) & 1);
+    const int row = site - xh;
+    const int sy = Lxh, sz = Lxh * Ly, st = Lxh * Ly * Lz;
+
+    int nbf[4], nbb[4];
+    nbf[0] = row + ((x + 1 == Lx ? 0 : x + 1) >> 1);
+    nbb[0] = row + ((x == 0 ? Lx - 1 : x - 1) >> 1);
+    nbf[1] = site + (y + 1 == Ly ? -(Ly - 1) * sy : sy);
+    nbb[1] = site + (y == 0 ? (Ly - 1) * sy : -sy);
+    nbf[2] = site + (z + 1 == Lz ? -(Lz - 1) * sz : sz);
+    nbb[2] = site + (z == 0 ? (Lz - 1) * sz : -sz);
+    nbf[3] = site + (t + 1 == Lt ? -(Lt - 1) * st : st);
+    nbb[3] = site + (t == 0 ? (Lt - 1) * st : -st);
+
+    const long site_len = 4L * NC * R;
+    const long link_len = long(NC) * NC;
+    const CS *inr = in + r;
+    const long Vh = g.Vh;
+
+    Cx<T> acc[4][NC];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[s][c] = {T(0), T(0)};
+
+#define QCU_HOP(MU)                                                                          \
+    hop<T, CS, CG, NC, MU, true, DAG>(acc, inr + nbf[MU] * site_len, R,                       \
+                                      gauge + ((MU * 2 + parity) * Vh + site) * link_len);    \
+    hop<T, CS, CG, NC, MU, false, DAG>(acc, inr + nbb[MU] * site_len, R,                      \
+                                       gauge + ((MU * 2 + 1 - parity) * Vh + nbb[MU]) * link_len);
+    QCU_HOP(0)
+    QCU_HOP(1)
+    QCU_HOP(2)
+    QCU_HOP(3)
+#undef QCU_HOP
+
+    CS *o = out + site * site_len + r;
+    if (xpay) {
+        const CS *xp = xpay + site * site_len + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                Cx<T> v = load_cx<T>(xp + (s * NC + c) * R);
+                v.re = fma(coef, acc[s][c].re, v.re);
+                v.im = fma(coef, acc[s][c].im, v.im);
+                store_cx(o + (s * NC + c) * R, v);
+            }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) store_cx(o + (s * NC + c) * R, coef * acc[s][c]);
+    }
+}
+
+template <typename T, typename S, int NC>
+hipError_t launch_valu(const DslashArgs &a, hipStream_t stream) {
+    using CS = typename CxStore<S>::type;
+    const int spb = a.R >= 256 ? 1 : 256 / a.R;
+    dim3 block(a.R, spb);
+    dim3 grid(div_up(a.g.Vh, spb));
+    auto out = static_cast<CS *>(a.out);
+    auto in = static_cast<const CS *>(a.in);
+    auto gauge = static_cast<const CS *>(a.gauge);
+    auto xp = static_cast<const CS *>(a.xpay);
+    if (a.dagger)
+        hipLaunchKernelGGL((k_wilson_dslash_valu<T, CS, CS, NC, true>), grid, block, 0, stream,
+                           out, in, gauge, xp, T(a.coef), a.g, a.R, a.parity);
+    else
+        hipLaunchKernelGGL((k_wilson_dslash_valu<T, CS, CS, NC, false>), grid, block, 0, stream,
+                           out, in, gauge, xp, T(a.coef), a.g, a.R, a.parity);
+    return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t dispatch_nc(const DslashArgs &a, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_valu<T, S, 2>(a, stream);
+        case 3: return launch_valu<T, S, 3>(a, stream);
+        case 4: return launch_valu<T, S, 4>(a, stream);
+        case 8: return launch_valu<T, S, 8>(a, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool dslash_supported(int Nc, int R) {
+    return (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8) && R >= 1 && R <= 256;
+}
+
+hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
+    if (!dslash_supported(a.Nc, a.R)) return hipErrorInvalidValue;
+    if (a.g.Vh <= 0) return hipSuccess;
+    switch (a.prec) {
+        case kDouble: return dispatch_nc<double, double>(a, stream);
+        case kSingle: return dispatch_nc<float, float>(a, stream);
+        case kHalf: return dispatch_nc<float, __half>(a, stream);
+        case kBF16: return dispatch_nc<float, __hip_bfloat16>(a, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace qcu

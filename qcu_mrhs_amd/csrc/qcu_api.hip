@@ -1,0 +1,494 @@
+// qcu_api.hip -- the C ABI (include/qcu.h, include/qcu_ops.h) over a
+// process-wide context, mirroring the reference's facade qcu::Qcu
+// (include/qcu_interface.h:12-113, src/qcu_interface.cu:22-413) and its global
+// singleton (src/interface.cu:12-18).
+//
+// Differences from the reference, by design:
+//  * one GPU: the process grid must be 1x1x1x1 and no MPI is needed
+//    (the reference asserts MPI_Comm_size == Gx*Gy*Gz*Gt, src/qcu_config/qcu_config.cu:76-86);
+//  * misuse sets an error code (qcuLastError) instead of aborting, unless
+//    QCU_ABORT_ON_ERROR=1;
+//  * the R fermion pointers travel by value in kernel arguments (the
+//    reference copies a device pointer table per call, src/qcu_interface.cu:123-126);
+//  * the solve starts from x0 = 0 (the reference's memset is a no-op,
+//    src/solver/bicgstab_solve.cu:243) and every call is stream ordered on one
+//    library stream with a single synchronisation at the end.
+#include <chrono>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "layout.h"
+#include "lgnn.h"
+#include "qcu.h"
+#include "qcu_ops.h"
+#include "solver.h"
+
+namespace qcu {
+
+namespace {
+int g_err = 0;
+char g_msg[1024] = "no error";
+}  // namespace
+
+void set_error(int code, const char *fmt, ...) {
+    g_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "[qcu] error %d: %s\n", code, g_msg);
+    const char *ab = getenv("QCU_ABORT_ON_ERROR");
+    if (ab && ab[0] == '1') abort();
+}
+
+int last_error() { return g_err; }
+
+struct Context {
+    Geom g{};
+    int Nc = 3, R = 1;
+    int user_prec = kDouble, iter_prec = kDouble;
+    double mass = 0.0, kappa = 0.125;
+    bool have_dslash = false;
+    void *gauge64 = nullptr;
+    void *gauge_lo[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *bufA = nullptr, *bufB = nullptr;  // full-lattice MRHS vectors, FP64-sized
+    std::vector<void *> q_in, q_out;
+    std::unique_ptr<BiCGStab> solver;
+    SolverOptions opts;
+    std::unique_ptr<LgnnPreconditioner> precond;
+    hipStream_t stream = nullptr;
+
+    long half_n() const { return long(g.Vh) * 4 * Nc; }
+    long gauge_count() const { return 4L * 2 * g.Vh * Nc * Nc; }
+
+    ~Context() {
+        solver.reset();
+        precond.reset();
+        for (void *p : {gauge64, bufA, bufB}) if (p) (void)hipFree(p);
+        for (void *p : gauge_lo) if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // gauge field in storage precision `prec` (converted lazily from FP64)
+    const void *gauge(int prec) {
+        if (prec == kDouble) return gauge64;
+        if (!gauge_lo[prec]) {
+            if (hipMalloc(&gauge_lo[prec], size_t(gauge_count()) * 2 * prec_bytes(prec)) != hipSuccess) {
+                set_error(kQcuErrorDevice, "cannot allocate gauge copy");
+                gauge_lo[prec] = nullptr;
+                return nullptr;
+            }
+            if (launch_convert(gauge_lo[prec], prec, gauge64, kDouble, gauge_count(), nullptr, 1,
+                               stream) != hipSuccess) {
+                set_error(kQcuErrorDevice, "gauge conversion failed");
+                return nullptr;
+            }
+        }
+        return gauge_lo[prec];
+    }
+};
+
+Context *g_ctx = nullptr;
+
+Context *ctx_or_error() {
+    if (!g_ctx) set_error(kQcuErrorNotInitialized, "Qcu is not initialized (call initGridSize first)");
+    return g_ctx;
+}
+
+bool valid_storage(int p) { return p == kHalf || p == kSingle || p == kDouble || p == kBF16; }
+
+bool queue_full(Context *c) {
+    if (int(c->q_in.size()) != c->R || int(c->q_out.size()) != c->R) {
+        set_error(kQcuErrorQueue, "fermion queue holds %zu pairs, expected %d", c->q_in.size(), c->R);
+        return false;
+    }
+    return true;
+}
+
+PtrTable table(const std::vector<void *> &v, long offset_elems, int prec) {
+    PtrTable t{};
+    const size_t eb = 2 * prec_bytes(prec);
+    for (size_t i = 0; i < v.size(); ++i)
+        t.p[i] = static_cast<const char *>(v[i]) + offset_elems * eb;
+    return t;
+}
+
+bool hip_ok(hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    if (last_error() == 0 || last_error() == kQcuErrorNotConverged)
+        set_error(kQcuErrorDevice, "%s: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+// stateless ops: one workspace for reductions up to 256 RHS
+ReduceWork g_ops_red;
+bool ops_red_ready() {
+    if (g_ops_red.partials) return true;
+    return hip_ok(reduce_work_alloc(g_ops_red, 256), "reduction workspace");
+}
+
+}  // namespace qcu
+
+using namespace qcu;
+
+extern "C" {
+
+int qcuLastError(void) { return g_err; }
+const char *qcuLastErrorString(void) { return g_msg; }
+void qcuClearError(void) {
+    g_err = 0;
+    strcpy(g_msg, "no error");
+}
+
+void initGridSize(QcuGrid *grid, QcuParam *param, int n_color, int m_rhs, int inputFloatPrecision,
+                  int dslashFloatPrecision) {
+    if (!grid || !param) return set_error(kQcuErrorInvalidValue, "null grid/param");
+    for (int i = 0; i < 4; ++i)
+        if (grid->grid_size[i] != 1)
+            return set_error(kQcuErrorUnsupported, "process grid must be 1x1x1x1 (one GPU)");
+    int L[4];
+    for (int i = 0; i < 4; ++i) {
+        L[i] = param->lattice_size[i];
+        if (L[i] <= 0) return set_error(kQcuErrorInvalidValue, "lattice size must be positive");
+    }
+    if (L[0] % 2) return set_error(kQcuErrorInvalidValue, "Lx must be even");
+    if (!dslash_supported(n_color, m_rhs) || m_rhs > kMaxPtrs)
+        return set_error(kQcuErrorUnsupported, "n_color %d / m_rhs %d not supported (Nc in {2,3,4,8}, 1<=R<=%d)",
+                         n_color, m_rhs, kMaxPtrs);
+    if (!(inputFloatPrecision == kSingle || inputFloatPrecision == kDouble))
+        return set_error(kQcuErrorUnsupported, "fermion precision %d not supported (Single or Double)",
+                         inputFloatPrecision);
+    if (!valid_storage(dslashFloatPrecision))
+        return set_error(kQcuErrorUnsupported, "dslash precision %d not supported", dslashFloatPrecision);
+    delete g_ctx;
+    g_ctx = nullptr;
+    auto *c = new Context();
+    c->g = make_geom(L);
+    c->Nc = n_color;
+    c->R = m_rhs;
+    c->user_prec = inputFloatPrecision;
+    c->iter_prec = dslashFloatPrecision;
+    c->opts.inner_prec = dslashFloatPrecision;
+    const size_t full = size_t(2) * c->half_n() * m_rhs * sizeof(Cx<double>);
+    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream") ||
+        !hip_ok(hipMalloc(&c->bufA, full), "alloc") || !hip_ok(hipMalloc(&c->bufB, full), "alloc") ||
+        !hip_ok(hipMalloc(&c->gauge64, size_t(c->gauge_count()) * sizeof(Cx<double>)), "alloc")) {
+        delete c;
+        return;
+    }
+    c->solver = std::make_unique<BiCGStab>(c->g, n_color, m_rhs);
+    g_ctx = c;
+}
+
+void pushBackFermions(void *fermionOut, void *fermionIn) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (int(c->q_in.size()) >= c->R) return set_error(kQcuErrorQueue, "Fermion queue is full");
+    c->q_in.push_back(fermionIn);
+    c->q_out.push_back(fermionOut);
+}
+
+void loadQcuGauge(void *gauge, int floatPrecision) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (!valid_storage(floatPrecision)) return set_error(kQcuErrorUnsupported, "gauge precision %d", floatPrecision);
+    for (auto &p : c->gauge_lo)
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    if (!hip_ok(launch_convert(c->gauge64, kDouble, gauge, floatPrecision, c->gauge_count(), nullptr, 1,
+                               c->stream), "loadQcuGauge"))
+        return;
+    (void)hip_ok(hipStreamSynchronize(c->stream), "loadQcuGauge");
+}
+
+void getDslash(int dslashType, double mass) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (dslashType != kQcuDslashWilson) return set_error(kQcuErrorUnsupported, "Unsupported dslash type %d", dslashType);
+    c->mass = mass;
+    c->kappa = 1.0 / (2.0 * (4.0 + mass));
+    c->have_dslash = true;
+}
+
+void start_dslash(int parity, int daggerFlag) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (!c->have_dslash) return set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
+    if (!queue_full(c)) return;
+    if (parity != 0 && parity != 1) return set_error(kQcuErrorInvalidValue, "parity must be 0 or 1");
+    const int P = c->iter_prec;
+    const long n = c->half_n();
+    bool ok = hip_ok(launch_gather(c->bufA, P, table(c->q_in, 0, c->user_prec), c->user_prec, n, c->R,
+                                   c->stream), "gather");
+    DslashArgs a{c->bufB, c->bufA, c->gauge(P), nullptr, 1.0, c->g, c->Nc, c->R, parity, daggerFlag ? 1 : 0, P};
+    ok = ok && a.gauge && hip_ok(launch_dslash(a, c->stream), "dslash");
+    ok = ok && hip_ok(launch_scatter(table(c->q_out, 0, c->user_prec), c->user_prec, c->bufB, P, n, c->R,
+                                     c->stream), "scatter");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "start_dslash");
+    c->q_in.clear();
+    c->q_out.clear();
+}
+
+void mat_Qcu(int daggerFlag) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (!c->have_dslash) return set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
+    if (!queue_full(c)) return;
+    const int P = c->iter_prec;
+    const long n = c->half_n();
+    const size_t eb = 2 * prec_bytes(P);
+    char *A = static_cast<char *>(c->bufA), *B = static_cast<char *>(c->bufB);
+    bool ok = hip_ok(launch_gather(A, P, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n, c->R,
+                                   c->stream), "gather");
+    for (int p = 0; p < 2 && ok; ++p) {
+        // out_p = in_p - kappa D_p in_{1-p}   (src/qcu_interface.cu:174-255)
+        DslashArgs a{B + p * n * c->R * eb, A + (1 - p) * n * c->R * eb, c->gauge(P), A + p * n * c->R * eb,
+                     -c->kappa, c->g, c->Nc, c->R, p, daggerFlag ? 1 : 0, P};
+        ok = a.gauge && hip_ok(launch_dslash(a, c->stream), "dslash");
+    }
+    ok = ok && hip_ok(launch_scatter(table(c->q_out, 0, c->user_prec), c->user_prec, B, P, 2 * n, c->R,
+                                     c->stream), "scatter");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "mat_Qcu");
+    c->q_in.clear();
+    c->q_out.clear();
+}
+
+int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
+    Context *c = ctx_or_error();
+    if (!c) return kQcuErrorNotInitialized;
+    if (!c->have_dslash) {
+        set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
+        return g_err;
+    }
+    if (!queue_full(c)) return g_err;
+    const auto t0 = std::chrono::steady_clock::now();
+    const long n = c->half_n();
+    SolveResult res;
+    GaugeSet gs;
+    gs.g64 = c->gauge64;
+    if (c->opts.inner_prec != kDouble) {
+        gs.glo = c->gauge(c->opts.inner_prec);
+        gs.lo_prec = c->opts.inner_prec;
+    }
+    Preconditioner *pc = nullptr;
+    if (c->opts.use_precond) {
+        if (!c->precond || !c->precond->ready()) {
+            set_error(kQcuErrorNotInitialized, "no learned preconditioner loaded");
+            c->q_in.clear();
+            c->q_out.clear();
+            return g_err;
+        }
+        pc = c->precond.get();
+    }
+    bool ok = hip_ok(launch_gather(c->bufA, kDouble, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n,
+                                   c->R, c->stream), "gather");
+    int code = kQcuSuccess;
+    if (ok) {
+        const hipError_t e = c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, c->opts, pc,
+                                              res, c->stream);
+        if (e != hipSuccess) {
+            ok = false;
+            code = g_err ? g_err : kQcuErrorDevice;
+        }
+    }
+    ok = ok && hip_ok(launch_scatter(table(c->q_out, 0, c->user_prec), c->user_prec, c->bufB, kDouble, 2 * n,
+                                     c->R, c->stream), "scatter");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "qcuInvert");
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->q_in.clear();
+    c->q_out.clear();
+    if (!ok && code == kQcuSuccess) code = g_err ? g_err : kQcuErrorDevice;
+    if (ok && !res.converged) {
+        set_error(kQcuErrorNotConverged, "BiCGStab did not converge in %d iterations (max rel residual %.3e)",
+                  res.iterations, res.true_rel_residual);
+        code = kQcuErrorNotConverged;
+    }
+    if (info) {
+        info->converged = res.converged;
+        info->iterations = res.iterations;
+        info->reliable_updates = res.reliable_updates;
+        info->restarts = res.restarts;
+        info->true_rel_residual = res.true_rel_residual;
+        info->seconds = secs;
+    }
+    return code;
+}
+
+void qcuInvert(int max_iteration, double p_max_prec) { (void)qcuInvertEx(max_iteration, p_max_prec, nullptr); }
+
+int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditioner) {
+    Context *c = ctx_or_error();
+    if (!c) return kQcuErrorNotInitialized;
+    if (!valid_storage(inner_precision)) {
+        set_error(kQcuErrorUnsupported, "inner precision %d", inner_precision);
+        return g_err;
+    }
+    if (!(delta > 0.0 && delta < 1.0)) {
+        set_error(kQcuErrorInvalidValue, "delta must be in (0, 1)");
+        return g_err;
+    }
+    c->opts.inner_prec = inner_precision;
+    c->opts.delta = delta;
+    c->opts.use_precond = use_preconditioner ? 1 : 0;
+    const char *v = getenv("QCU_VERBOSE");
+    c->opts.verbose = (v && v[0] == '1') ? 1 : 0;
+    return kQcuSuccess;
+}
+
+void finalizeQcu(void) {
+    delete g_ctx;
+    g_ctx = nullptr;
+    reduce_work_free(g_ops_red);
+}
+
+void gauge_eo_precondition(void *prec_gauge, void *non_prec_gauge, int precision) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (prec_gauge == non_prec_gauge) return set_error(kQcuErrorInvalidValue, "in-place conversion not supported");
+    if (!valid_storage(precision)) return set_error(kQcuErrorUnsupported, "UNDEFINED precision");
+    if (hip_ok(launch_gauge_eo(prec_gauge, non_prec_gauge, c->g, c->Nc, precision, 0, c->stream), "eo"))
+        (void)hip_ok(hipStreamSynchronize(c->stream), "eo");
+}
+
+void gauge_reverse_eo_precondition(void *non_prec_gauge, void *prec_gauge, int precision) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (prec_gauge == non_prec_gauge) return set_error(kQcuErrorInvalidValue, "in-place conversion not supported");
+    if (!valid_storage(precision)) return set_error(kQcuErrorUnsupported, "UNDEFINED precision");
+    if (hip_ok(launch_gauge_eo(prec_gauge, non_prec_gauge, c->g, c->Nc, precision, 1, c->stream), "eo"))
+        (void)hip_ok(hipStreamSynchronize(c->stream), "eo");
+}
+
+void read_gauge_from_file(void *gauge, const char *file_path_prefix) {
+    Context *c = ctx_or_error();
+    if (!c) return;
+    if (!file_path_prefix) return set_error(kQcuErrorInvalidValue, "null path");
+    FILE *f = fopen(file_path_prefix, "rb");
+    if (!f) return set_error(kQcuErrorInvalidValue, "cannot open %s", file_path_prefix);
+    const size_t count = size_t(c->gauge_count());
+    std::vector<Cx<double>> host(count);
+    const size_t got = fread(host.data(), sizeof(Cx<double>), count, f);
+    fclose(f);
+    if (got != count)
+        return set_error(kQcuErrorInvalidValue, "%s holds %zu complex values, expected %zu", file_path_prefix, got,
+                         count);
+    void *lex = nullptr;
+    if (!hip_ok(hipMalloc(&lex, count * sizeof(Cx<double>)), "alloc")) return;
+    bool ok = hip_ok(hipMemcpy(lex, host.data(), count * sizeof(Cx<double>), hipMemcpyHostToDevice), "copy");
+    ok = ok && hip_ok(launch_gauge_eo(gauge, lex, c->g, c->Nc, kDouble, 0, c->stream), "eo");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "eo");
+    (void)hipFree(lex);
+}
+
+// ---- device memory plumbing ---------------------------------------------
+void *qcuDeviceMalloc(size_t bytes) {
+    void *p = nullptr;
+    if (!hip_ok(hipMalloc(&p, bytes), "qcuDeviceMalloc")) return nullptr;
+    return p;
+}
+int qcuDeviceFree(void *ptr) { return hip_ok(hipFree(ptr), "qcuDeviceFree") ? 0 : kQcuErrorDevice; }
+int qcuMemcpyHtoD(void *dst, const void *src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "HtoD") ? 0 : kQcuErrorDevice;
+}
+int qcuMemcpyDtoH(void *dst, const void *src, size_t bytes) {
+    return hip_ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "DtoH") ? 0 : kQcuErrorDevice;
+}
+int qcuMemsetD(void *dst, int value, size_t bytes) {
+    return hip_ok(hipMemset(dst, value, bytes), "memset") ? 0 : kQcuErrorDevice;
+}
+int qcuDeviceSynchronize(void) { return hip_ok(hipDeviceSynchronize(), "sync") ? 0 : kQcuErrorDevice; }
+
+// ---- stateless operations (qcu_ops.h) -----------------------------------
+int qcuDslashMrhs(void *out, const void *in, const void *gauge, const int lattice[4], int n_color, int m_rhs,
+                  int parity, int dagger, int precision, const void *xpay, double coef) {
+    if (!lattice || lattice[0] % 2 || !dslash_supported(n_color, m_rhs) || !valid_storage(precision) ||
+        (parity != 0 && parity != 1)) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashMrhs: invalid arguments");
+        return g_err;
+    }
+    DslashArgs a{out, in, gauge, xpay, coef, make_geom(lattice), n_color, m_rhs, parity, dagger ? 1 : 0, precision};
+    return hip_ok(launch_dslash(a, nullptr), "qcuDslashMrhs") ? 0 : g_err;
+}
+
+static int blas_axpy(int kind, void *res, const void *x, const void *y, const void *z, const void *a,
+                     const void *b, const void *cc, long n, int R, int prec) {
+    if (!valid_storage(prec) || R < 1 || R > 256 || n < 0) {
+        set_error(kQcuErrorInvalidValue, "blas: invalid arguments");
+        return g_err;
+    }
+    AxpyArgs ax{kind, res, x, y, z, static_cast<const C64 *>(a), static_cast<const C64 *>(b),
+                static_cast<const C64 *>(cc), n, R, prec};
+    return hip_ok(launch_axpy(ax, nullptr), "blas") ? 0 : g_err;
+}
+
+int qcuBlasXsay(void *res, const void *x, const void *a, const void *y, long n, int m_rhs, int precision) {
+    return blas_axpy(kXsay, res, x, y, nullptr, a, nullptr, nullptr, n, m_rhs, precision);
+}
+int qcuBlasXpay(void *res, const void *x, const void *a, const void *y, long n, int m_rhs, int precision) {
+    return blas_axpy(kXpay, res, x, y, nullptr, a, nullptr, nullptr, n, m_rhs, precision);
+}
+int qcuBlasAxpby(void *res, const void *a, const void *x, const void *b, const void *y, long n, int m_rhs,
+                 int precision) {
+    return blas_axpy(kAxpby, res, x, y, nullptr, a, b, nullptr, n, m_rhs, precision);
+}
+int qcuBlasAxpbypcz(void *res, const void *a, const void *x, const void *b, const void *y, const void *c,
+                    const void *z, long n, int m_rhs, int precision) {
+    return blas_axpy(kAxpbypcz, res, x, y, z, a, b, c, n, m_rhs, precision);
+}
+int qcuBlasDotc(void *res, const void *a, const void *b, long n, int m_rhs, int precision) {
+    if (!valid_storage(precision) || m_rhs < 1 || m_rhs > 256 || !ops_red_ready()) {
+        set_error(kQcuErrorInvalidValue, "qcuBlasDotc: invalid arguments");
+        return g_err;
+    }
+    return hip_ok(launch_dotc(static_cast<C64 *>(res), a, b, n, m_rhs, precision, g_ops_red, 14, nullptr), "dotc")
+               ? 0 : g_err;
+}
+int qcuBlasNorm(void *res, const void *a, long n, int m_rhs, int precision) {
+    if (!valid_storage(precision) || m_rhs < 1 || m_rhs > 256 || !ops_red_ready()) {
+        set_error(kQcuErrorInvalidValue, "qcuBlasNorm: invalid arguments");
+        return g_err;
+    }
+    return hip_ok(launch_norm(static_cast<double *>(res), a, n, m_rhs, precision, true, g_ops_red, 15, nullptr),
+                  "norm") ? 0 : g_err;
+}
+
+int qcuColorSpinorGather(void *dst, int dst_precision, void *const *src_ptrs, int src_precision, long nsites,
+                         int n_color, int m_rhs) {
+    if (!src_ptrs || m_rhs < 1 || m_rhs > kMaxPtrs || !valid_storage(dst_precision) ||
+        !valid_storage(src_precision)) {
+        set_error(kQcuErrorInvalidValue, "gather: invalid arguments");
+        return g_err;
+    }
+    PtrTable t{};
+    for (int i = 0; i < m_rhs; ++i) t.p[i] = src_ptrs[i];
+    return hip_ok(launch_gather(dst, dst_precision, t, src_precision, nsites * 4 * n_color, m_rhs, nullptr),
+                  "gather") ? 0 : g_err;
+}
+int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *src, int src_precision,
+                          long nsites, int n_color, int m_rhs) {
+    if (!dst_ptrs || m_rhs < 1 || m_rhs > kMaxPtrs || !valid_storage(dst_precision) ||
+        !valid_storage(src_precision)) {
+        set_error(kQcuErrorInvalidValue, "scatter: invalid arguments");
+        return g_err;
+    }
+    PtrTable t{};
+    for (int i = 0; i < m_rhs; ++i) t.p[i] = dst_ptrs[i];
+    return hip_ok(launch_scatter(t, dst_precision, src, src_precision, nsites * 4 * n_color, m_rhs, nullptr),
+                  "scatter") ? 0 : g_err;
+}
+int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_precision, long count) {
+    if (!valid_storage(dst_precision) || !valid_storage(src_precision) || count < 0) {
+        set_error(kQcuErrorInvalidValue, "copy: invalid arguments");
+        return g_err;
+    }
+    return hip_ok(launch_convert(dst, dst_precision, src, src_precision, count, nullptr, 1, nullptr), "copy")
+               ? 0 : g_err;
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// solver.h -- even/odd preconditioned multi-RHS BiCGStab on gfx950.
+#pragma once
+
+#include <vector>
+
+#include "blas.h"
+#include "dslash.h"
+
+namespace qcu {
+
+// Right preconditioner N for the odd Schur system: the solver iterates on
+// A N y = b' and returns x = N y.
+class Preconditioner {
+public:
+    virtual ~Preconditioner() = default;
+    // out = N in on odd-parity MRHS half vectors stored in precision `prec`
+    virtual hipError_t apply(void *out, const void *in, int prec, hipStream_t s) = 0;
+    virtual bool supports(int prec) const = 0;
+};
+
+struct SolverOptions {
+    int inner_prec = kDouble;  // kDouble: plain FP64 BiCGStab; else reliable-update mixed precision
+    double delta = 0.1;        // reliable update when ||r|| < delta * max ||r|| since last update
+    int use_precond = 0;       // apply the learned right preconditioner (inner solve only)
+    int verbose = 0;
+};
+
+struct SolveResult {
+    int converged = 0;
+    int iterations = 0;
+    int reliable_updates = 0;
+    int restarts = 0;
+    double true_rel_residual = -1.0;  // max_r ||b - M x|| / ||b|| over the full lattice
+    std::vector<double> rel_residual;  // per RHS, odd Schur system, last estimate
+};
+
+// Device fields the solver reads: the FP64 gauge field and (for a mixed solve)
+// a copy in the inner precision.
+struct GaugeSet {
+    const void *g64 = nullptr;
+    const void *glo = nullptr;
+    int lo_prec = kDouble;
+};
+
+class BiCGStab {
+public:
+    BiCGStab(const Geom &g, int Nc, int R);
+    ~BiCGStab();
+    BiCGStab(const BiCGStab &) = delete;
+    BiCGStab &operator=(const BiCGStab &) = delete;
+
+    // b_full, x_full: FP64 MRHS full vectors [2][Vh][4][Nc][R] on the device.
+    hipError_t solve(const void *b_full, void *x_full, const GaugeSet &gauge, double kappa,
+                     int max_iter, double tol, const SolverOptions &opt, Preconditioner *prec,
+                     SolveResult &res, hipStream_t s);
+
+    // FP64 full-lattice true residual max_r ||b - M x|| / ||b||.
+    hipError_t true_residual(const void *b_full, const void *x_full, const void *g64, double kappa,
+                             std::vector<double> &rel, hipStream_t s);
+
+    // out = in - kappa^2 D_oe D_eo in in precision `prec` (tmp: half vector)
+    hipError_t apply_schur(void *out, const void *in, void *tmp, const void *gauge, int prec,
+                           double kappa, int dagger, hipStream_t s);
+
+    const Geom &geom() const { return g_; }
+    int nc() const { return Nc_; }
+    int nrhs() const { return R_; }
+
+private:
+    hipError_t ensure(int prec);
+    hipError_t dslash(void *out, const void *in, const void *gauge, int prec, int parity,
+                      const void *xpay, double coef, int dagger, hipStream_t s);
+
+    Geom g_;
+    int Nc_, R_;
+    long n_;        // complex entries per RHS in a half vector (Vh * 4 * Nc)
+    int alloc_prec_ = -1;
+    // inner (sloppy) vectors
+    void *r0_ = nullptr, *r_ = nullptr, *p_ = nullptr, *v_ = nullptr, *s_ = nullptr, *t_ = nullptr,
+         *tmp_ = nullptr, *xlo_ = nullptr, *ph_ = nullptr, *sh_ = nullptr;
+    // FP64 vectors
+    void *bp_ = nullptr, *xhi_ = nullptr, *rhi_ = nullptr, *tmp64_ = nullptr, *v64_ = nullptr;
+    C64 *st_ = nullptr;        // scalar slots [kNumSlots][R]
+    C64 *ones_ = nullptr;      // [R] of 1
+    C64 *kap_ = nullptr;       // [R] of kappa
+    double *dn_ = nullptr;     // [4][R] norms
+    double *dscale_ = nullptr; // [R] per-RHS scale
+    double *h_ = nullptr;      // pinned host staging [8][R]
+    ReduceWork red_;
+};
+
+}  // namespace qcu
