@@ -1,0 +1,274 @@
+#!/usr/bin/env python3
+"""Time-to-solution and dslash bandwidth of the multi-RHS Wilson solver on one MI355X.
+
+Workload (default): SU(3) Wilson, 24^3 x 48 lattice, 16 right-hand sides,
+kappa = 0.1355, links exp(i 0.5 H) (H traceless Hermitian Gaussian), Gaussian
+sources, even/odd preconditioned BiCGStab to a true relative residual of 1e-10.
+
+A step is one full multi-RHS solve through the C entry points
+(pushBackFermions x 16 + qcuInvertEx), inputs resident in HBM.  `value` is
+the mean wall time of the headline method over the timed steps; the plain
+FP64 solve on the same GPU is timed beside it, together with the per-precision
+dslash bandwidth (HIP events) and a CPU baseline (the FP64 oracle on a bounded
+sample of the same system).
+
+    python bench.py [--steps K] [--warmup W] [--workload c3|c2|small] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from qcu_mrhs_amd import api, fields  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+PREC = {"double": 2, "single": 1, "half": 0, "bf16": 4}
+CPLX_BYTES = {2: 16, 1: 8, 0: 4, 4: 4}
+
+WORKLOADS = {
+    "c3": dict(L=(24, 24, 24, 48), Nc=3, R=16, kappa=0.1355, kind="smooth", eps=0.5, seeds=(3001, 3002),
+               name="SU(3) Wilson 24^3x48, 16 RHS, kappa 0.1355, exp(i 0.5 H) links, tol 1e-10"),
+    "c2": dict(L=(16, 16, 16, 32), Nc=3, R=12, kappa=0.135, kind="smooth", eps=0.3, seeds=(2001, 2002),
+               name="SU(3) Wilson 16^3x32, 12 RHS, kappa 0.135, exp(i 0.3 H) links, tol 1e-10"),
+    "small": dict(L=(8, 8, 8, 8), Nc=3, R=4, kappa=0.12, kind="haar", eps=0.0, seeds=(1001, 1002),
+                  name="SU(3) Wilson 8^3x8, 4 RHS, kappa 0.12, Haar links, tol 1e-10"),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def dslash_bytes(Vh, Nc, R, cplx_bytes):
+    """Compulsory HBM bytes of one one-parity dslash: 8 links per site
+    (4 forward at x, 4 backward at x-mu: every link once), the input spinor
+    once, the output spinor once."""
+    return Vh * (8 * Nc * Nc * cplx_bytes + 2 * 4 * Nc * R * cplx_bytes)
+
+
+def dslash_flops(Vh, Nc, R):
+    return Vh * R * (8 * 2 * (8 * Nc - 2) * Nc + 8 * 2 * 2 * Nc + 7 * 8 * Nc)
+
+
+def time_dslash(L, Nc, R, d_src_even, d_gauge64, prec, reps=20):
+    """Average kernel time (ms) of the one-parity dslash in storage precision
+    `prec`, HIP events on the stream the kernel runs on."""
+    Vh = fields.half_volume(L)
+    n = Vh * 4 * Nc * R
+    cb = CPLX_BYTES[prec]
+    d_in = api.DeviceArray(n * cb)
+    d_out = api.DeviceArray(n * cb)
+    d_g = api.DeviceArray(4 * 2 * Vh * Nc * Nc * cb)
+    L_ = api.lib()
+    api.check(L_.qcuCopyComplexVector(d_in.ptr, prec, d_src_even.ptr, 2, n))
+    api.check(L_.qcuCopyComplexVector(d_g.ptr, prec, d_gauge64.ptr, 2, 4 * 2 * Vh * Nc * Nc))
+    for _ in range(3):
+        api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, 1, 0, precision=prec)
+    api.synchronize()
+    api.check(L_.qcuTimerStart())
+    for _ in range(reps):
+        api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, 1, 0, precision=prec)
+    ms = L_.qcuTimerStopMs() / reps
+    for d in (d_in, d_out, d_g):
+        d.free()
+    return ms
+
+
+def solve_once(d_b, d_x, R, max_iter, tol):
+    for r in range(R):
+        api.pushBackFermions(d_x[r], d_b[r])
+    t0 = time.perf_counter()
+    info = api.qcuInvertEx(max_iter, tol)
+    info["wall_s"] = time.perf_counter() - t0
+    return info
+
+
+def cpu_baseline(w, gauge, b, plain_iters):
+    """FP64 oracle (OpenMP C) on the same system: BiCGStab with 1 and 2
+    iterations; the difference is the per-iteration cost, extrapolated to the
+    GPU plain-FP64 iteration count."""
+    from oracle import oracle
+
+    L, Nc, R, kappa = w["L"], w["Nc"], w["R"], w["kappa"]
+    bm = fields.to_mrhs(b, Nc)
+    t = []
+    for iters in (1, 2):
+        t0 = time.perf_counter()
+        oracle.bicgstab(bm, gauge, L, Nc, R, kappa, iters, 1e-300)
+        t.append(time.perf_counter() - t0)
+        log(f"cpu oracle bicgstab {iters} iteration(s): {t[-1]:.2f} s")
+    per_iter = max(t[1] - t[0], 1e-9)
+    setup = max(t[0] - per_iter, 0.0)
+    return {
+        "value": setup + per_iter * plain_iters,
+        "unit": "s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"FP64 C oracle BiCGStab on the full {w['name']} system, 1 and 2 iterations "
+                   f"({t[0]:.2f} s, {t[1]:.2f} s); time-to-solution extrapolated to the GPU plain "
+                   f"FP64 iteration count ({plain_iters})"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--methods", default="half,bf16,single",
+                    help="mixed inner precisions to time (headline = fastest converged)")
+    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--dslash-only", action="store_true",
+                    help="time only the dslash (all storage precisions) and a plain copy kernel "
+                         "of one half vector (counter calibration)")
+    ap.add_argument("--dslash-variant", default="auto", choices=["auto", "lane_rhs", "wave", "mfma"],
+                    help="dslash kernel variant for every launch (default: automatic choice)")
+    ap.add_argument("--dslash-precs", default="double,single,half,bf16",
+                    help="storage precisions timed by the dslash measurement")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (multi-GPU sharding is "
+              "design-only for this solver)", file=sys.stderr)
+        sys.exit(2)
+
+    w = WORKLOADS[args.workload]
+    L, Nc, R, kappa = w["L"], w["Nc"], w["R"], w["kappa"]
+    Vh = fields.half_volume(L)
+    log(f"workload {w['name']}; generating fields")
+    t0 = time.perf_counter()
+    gauge = fields.gauge_field(L, Nc, w["seeds"][0], kind=w["kind"], eps=w["eps"])
+    b = fields.gaussian_sources(L, Nc, R, w["seeds"][1])
+    log(f"fields ready in {time.perf_counter() - t0:.1f} s")
+
+    d_gauge = api.DeviceArray.from_complex(gauge.reshape(-1))
+    d_b = [api.DeviceArray.from_complex(b[r]) for r in range(R)]
+    d_x = [api.DeviceArray(b[r].nbytes) for r in range(R)]
+
+    # even halves gathered into the multi-RHS layout (dslash input)
+    import ctypes
+    half_bytes = Vh * 4 * Nc * 16
+    d_src = api.DeviceArray(Vh * 4 * Nc * R * 16)
+    ptrs = (ctypes.c_void_p * R)(*[d.ptr for d in d_b])
+    api.check(api.lib().qcuColorSpinorGather(d_src.ptr, 2, ptrs, 2, Vh, Nc, R))
+
+    api.dslash_tune(args.dslash_variant)
+    dslash = {}
+    for name, p in [(n, PREC[n]) for n in args.dslash_precs.split(",") if n]:
+        ms = time_dslash(L, Nc, R, d_src, d_gauge, p)
+        by = dslash_bytes(Vh, Nc, R, CPLX_BYTES[p])
+        dslash[name] = {"ms": ms, "GBps": by / ms / 1e6, "bytes": by,
+                        "frac_of_8TBps": by / ms / 1e6 / HBM_PEAK_GBPS,
+                        "TFLOPs": dslash_flops(Vh, Nc, R) / ms / 1e9}
+        log(f"dslash {name}: {ms * 1e3:.1f} us, {dslash[name]['GBps']:.0f} GB/s "
+            f"({100 * dslash[name]['frac_of_8TBps']:.1f}% of 8 TB/s)")
+    if args.dslash_only:
+        n = Vh * 4 * Nc * R
+        d_dst = api.DeviceArray(n * 16)
+        api.check(api.lib().qcuCopyComplexVector(d_dst.ptr, 2, d_src.ptr, 2, n))
+        api.synchronize()
+        api.check(api.lib().qcuTimerStart())
+        for _ in range(10):
+            api.check(api.lib().qcuCopyComplexVector(d_dst.ptr, 2, d_src.ptr, 2, n))
+        ms = api.lib().qcuTimerStopMs() / 10
+        copy = {"bytes_read": n * 16, "bytes_written": n * 16, "ms": ms, "GBps": 2 * n * 16 / ms / 1e6}
+        log(f"copy kernel: {ms * 1e3:.1f} us, {copy['GBps']:.0f} GB/s")
+        print(json.dumps({"workload": w["name"], "dslash": dslash, "copy": copy}))
+        return
+    d_src.free()
+    del half_bytes
+
+    api.initGridSize(L, Nc, R, 2, 2)
+    api.loadQcuGauge(d_gauge, 2)
+    api.getDslash(0, 1.0 / (2.0 * kappa) - 4.0)
+
+    results = {}
+    plain = None
+    if not args.no_plain:
+        api.set_solver_options(2, args.delta, False)
+        info = solve_once(d_b, d_x, R, args.max_iter, args.tol)
+        log(f"plain FP64: {info['wall_s']:.3f} s, {info['iterations']} iterations, "
+            f"true residual {info['true_rel_residual']:.2e}")
+        plain = info
+        results["plain_fp64"] = info
+
+    timed = {}
+    for m in [s for s in args.methods.split(",") if s]:
+        api.set_solver_options(PREC[m], args.delta, False)
+        for _ in range(args.warmup):
+            solve_once(d_b, d_x, R, args.max_iter, args.tol)
+        runs = [solve_once(d_b, d_x, R, args.max_iter, args.tol) for _ in range(args.steps)]
+        ok = all(r["converged"] and r["true_rel_residual"] <= args.tol for r in runs)
+        mean = float(np.mean([r["wall_s"] for r in runs]))
+        timed[m] = {"mean_s": mean, "ok": ok, "runs": runs}
+        log(f"mixed {m}: {mean:.3f} s/solve over {args.steps}, {runs[-1]['iterations']} iterations, "
+            f"{runs[-1]['reliable_updates']} reliable updates, true residual "
+            f"{max(r['true_rel_residual'] for r in runs):.2e}, converged={ok}")
+    api.finalizeQcu()
+
+    good = {m: v for m, v in timed.items() if v["ok"]}
+    if good:
+        head = min(good, key=lambda m: good[m]["mean_s"])
+        value = good[head]["mean_s"]
+        head_prec = PREC[head]
+        method = f"reliable-update mixed precision, {head} inner dslash/vectors, FP64 reductions/updates"
+    else:
+        head, value, head_prec = "plain_fp64", plain["wall_s"] if plain else None, 2
+        method = "plain FP64 BiCGStab"
+
+    dom = {2: "double", 1: "single", 0: "half", 4: "bf16"}[head_prec]
+    if dom not in dslash:
+        dom = next(iter(dslash))
+    roof = {"bound": "hbm", "achieved": dslash[dom]["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": dslash[dom]["frac_of_8TBps"], "traffic": None,
+            "kernel": f"k_wilson_dslash ({dom} storage)", "algorithmic_bytes": dslash[dom]["bytes"],
+            "avg_ms": dslash[dom]["ms"]}
+
+    cpu = None
+    if not args.no_cpu and plain is not None:
+        cpu = cpu_baseline(w, gauge, b, plain["iterations"])
+
+    out = {
+        "metric": "time-to-solution @1e-10 residual, mixed+learned vs plain FP64; dslash GB/s",
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": value * 1e3 if value else None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16/bf16/f32 storage, f32 arithmetic inner; f64 reductions and reliable updates"
+                 if head != "plain_fp64" else "f64",
+        "data": "synthetic (seeded exp(i eps H) links, Gaussian sources)",
+        "config": {"workload": w["name"], "lattice": list(L), "n_color": Nc, "m_rhs": R, "kappa": kappa,
+                   "tol": args.tol, "method": method, "headline": head},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_fp64_s": plain["wall_s"] if plain else None,
+        "plain_fp64_iterations": plain["iterations"] if plain else None,
+        "speedup_vs_plain_fp64": (plain["wall_s"] / value) if (plain and value) else None,
+        "methods": {m: {"mean_s": v["mean_s"], "ok": v["ok"], "iterations": v["runs"][-1]["iterations"],
+                        "reliable_updates": v["runs"][-1]["reliable_updates"],
+                        "true_rel_residual": max(r["true_rel_residual"] for r in v["runs"])}
+                    for m, v in timed.items()},
+        "dslash": dslash,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

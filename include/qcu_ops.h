@@ -59,6 +59,20 @@ int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *
 int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_precision,
                          long count);
 
+/* Launch shape of the dslash for every later call in this process (the
+ * stateless qcuDslashMrhs and the solver alike): variant 0 = automatic
+ * (tuning table, else MFMA for FP16/BF16 and wave-per-site VALU otherwise),
+ * 1 = lane per (site, rhs), 2 = wavefront per site (VALU), 3 = wavefront per
+ * site on MFMA (FP16/BF16 only); by, bz = traversal tile in y and z and spw =
+ * sites per wavefront (0 = choose).  No reference counterpart. */
+int qcuDslashTune(int variant, int by, int bz, int spw);
+
+/* Device timing of the stateless operations: HIP events recorded on the
+ * stream those operations run on.  qcuTimerStopMs waits for the stop event
+ * and returns the elapsed milliseconds (negative on error). */
+int qcuTimerStart(void);
+double qcuTimerStopMs(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,9 @@ def load() -> ctypes.CDLL:
         "qcuColorSpinorGather": ([vp, i, ctypes.POINTER(vp), i, l, i, i], i),
         "qcuColorSpinorScatter": ([ctypes.POINTER(vp), i, vp, i, l, i, i], i),
         "qcuCopyComplexVector": ([vp, i, vp, i, l], i),
+        "qcuDslashTune": ([i, i, i, i], i),
+        "qcuTimerStart": ([], i),
+        "qcuTimerStopMs": ([], d),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -225,6 +225,15 @@ def read_gauge_from_file(gauge, path):
 
 
 # --------------------------------------------------------- stateless ops
+DSLASH_VARIANT = {"auto": 0, "lane_rhs": 1, "wave": 2, "mfma": 3}
+
+
+def dslash_tune(variant="auto", by=0, bz=0, spw=0):
+    """Force the dslash launch shape for every later call (variant 'auto' resets)."""
+    v = DSLASH_VARIANT[variant] if isinstance(variant, str) else int(variant)
+    check(lib().qcuDslashTune(v, int(by), int(bz), int(spw)))
+
+
 def dslash_mrhs(out, inp, gauge, lattice, n_color, m_rhs, parity, dagger=0, precision=2,
                 xpay=None, coef=1.0):
     L4 = (ctypes.c_int * 4)(*lattice)

@@ -5,6 +5,21 @@
 
 namespace qcu {
 
+// Kernel variants of the one-parity hopping term.
+enum DslashVariant : int {
+    kDslashAuto = 0,      // choose per precision (and from the tuning table)
+    kDslashLaneRhs = 1,   // one lane per (site, rhs), all 8 hops per lane
+    kDslashWaveSite = 2,  // one wavefront per site: lanes = 16 rhs x 4 dimensions (VALU)
+    kDslashMfma = 3       // one wavefront per site on MFMA (FP16 / BF16 storage)
+};
+
+// Launch shape of the wave-per-site kernels.
+struct DslashTune {
+    int variant = kDslashAuto;
+    int by = 0, bz = 0;   // traversal tile in y and z (0 = choose)
+    int spw = 0;          // sites per wavefront (0 = choose)
+};
+
 struct DslashArgs {
     void *out;          // MRHS half vector, output parity
     const void *in;     // MRHS half vector, parity 1 - parity
@@ -17,9 +32,22 @@ struct DslashArgs {
     int parity;         // output parity
     int dagger;
     int prec;           // storage precision of out/in/gauge/xpay (Prec)
+    DslashTune tune{};
 };
 
 bool dslash_supported(int Nc, int R);
 hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream);
+
+// resolved launch shape (after defaults / tuning table)
+DslashTune dslash_resolve(const DslashArgs &a);
+// per-(lattice, Nc, R, prec) tuning table consulted when tune.variant == auto
+void dslash_set_tuned(const int L[4], int Nc, int R, int prec, const DslashTune &t);
+// process-wide override (variant != auto forces that shape for every launch)
+void dslash_set_override(const DslashTune &t);
+
+// variant launchers (dslash_wave.hip, dslash_mfma.hip)
+hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
+hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
+bool dslash_mfma_supported(int Nc, int prec);
 
 }  // namespace qcu

@@ -7,30 +7,20 @@
 // two projected rows per hop: h_r = psi_r + s_r psi_m(r), out_r += U h_r,
 // out_m(r) += c_r U h_r (include/kernel/qcu_gamma.cuh:47-324).
 //
-// VALU kernel: one lane per (site, rhs).  The RHS-innermost layout makes the
-// R lanes of one site read R consecutive complex numbers for every
-// (spin, colour) -- coalesced 16-byte loads; the link is a broadcast load
-// shared by those lanes.  Workgroup = R x (256/R) lanes, blocks remapped so
-// every XCD sweeps a contiguous slab of sites (neighbour reuse in its L2).
-// An optional epilogue fuses out = x + coef * D psi (Schur operator,
-// full-lattice M, source preparation).
-#include "dslash.h"
+// This file holds the dispatcher and the lane-per-(site, rhs) VALU kernel
+// (variant kDslashLaneRhs): R lanes of one site read R consecutive complex
+// numbers for every (spin, colour); the link is a broadcast load shared by
+// those lanes.  The wave-per-site kernels live in dslash_wave.hip (VALU) and
+// dslash_mfma.hip (matrix cores).  Every variant has an optional epilogue
+// out = x + coef * D psi (Schur operator, full-lattice M, source preparation).
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "dslash_common.h"
 
 namespace qcu {
 namespace {
-
-// projection / reconstruction tables, expressed as powers of i for the
-// (1 + gamma) form; the forward hop and the dagger each add a factor -1 = i^2.
-// mat2 (qcu_gamma.cuh:47-51,101-106,159-163,208-212), projection scale
-// (:61-67,115-123,172-180,222-228), reconstruction scale (:52-59,107-114,
-// 164-171,213-220).
-__host__ __device__ constexpr int kMat2(int mu, int r) { return mu < 2 ? 3 - r : 2 + r; }
-__host__ __device__ constexpr int kProjK(int mu, int r) {
-    return mu == 0 ? 1 : mu == 1 ? (r == 0 ? 2 : 0) : mu == 2 ? (r == 0 ? 1 : 3) : 0;
-}
-__host__ __device__ constexpr int kReconK(int mu, int r) {
-    return mu == 0 ? 3 : mu == 1 ? (r == 0 ? 2 : 0) : mu == 2 ? (r == 0 ? 3 : 1) : 0;
-}
 
 template <typename T, typename CS, typename CG, int NC, int MU, bool FWD, bool DAG>
 __device__ __forceinline__ void hop(Cx<T> (&acc)[4][NC], const CS *__restrict__ psi, int R,
@@ -181,9 +171,67 @@ bool dslash_supported(int Nc, int R) {
     return (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8) && R >= 1 && R <= 256;
 }
 
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int, int, int>, DslashTune> g_tuned;
+DslashTune g_override;  // variant != auto: forced for every launch
+
+int largest_divisor_at_most(int L, int cap) {
+    for (int d = cap < L ? cap : L; d > 1; --d)
+        if (L % d == 0) return d;
+    return 1;
+}
+}  // namespace
+
+void dslash_set_tuned(const int L[4], int Nc, int R, int prec, const DslashTune &t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(L[0], L[1], L[2], L[3], Nc, R, prec)] = t;
+}
+
+void dslash_set_override(const DslashTune &t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_override = t;
+}
+
+DslashTune dslash_resolve(const DslashArgs &a) {
+    DslashTune t = a.tune;
+    if (t.variant == kDslashAuto) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_override.variant != kDslashAuto) {
+            t = g_override;
+        } else {
+            auto it = g_tuned.find(std::make_tuple(a.g.L[0], a.g.L[1], a.g.L[2], a.g.L[3], a.Nc, a.R, a.prec));
+            if (it != g_tuned.end()) t = it->second;
+        }
+    }
+    if (t.variant == kDslashAuto) t.variant = dslash_mfma_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
+    if (t.variant == kDslashMfma && !dslash_mfma_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
+    if (t.variant != kDslashLaneRhs) {
+        if (t.spw <= 0) t.spw = 4;
+        // tile the (y, z) plane so one tile slice (the t-reuse distance) stays
+        // around 1 MiB: bytes per site = 4 Nc R complex numbers
+        const double site_bytes = 4.0 * a.Nc * a.R * 2.0 * double(prec_bytes(a.prec));
+        double area = (1 << 20) / (site_bytes * a.g.Lxh);
+        if (area < 1) area = 1;
+        if (t.by <= 0 || a.g.L[1] % t.by) {
+            int cap = 1;
+            while ((cap + 1) * (cap + 1) <= area) ++cap;
+            t.by = largest_divisor_at_most(a.g.L[1], cap);
+        }
+        if (t.bz <= 0 || a.g.L[2] % t.bz) {
+            int cap = int(area / t.by);
+            t.bz = largest_divisor_at_most(a.g.L[2], cap < 1 ? 1 : cap);
+        }
+    }
+    return t;
+}
+
 hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
     if (!dslash_supported(a.Nc, a.R)) return hipErrorInvalidValue;
     if (a.g.Vh <= 0) return hipSuccess;
+    const DslashTune t = dslash_resolve(a);
+    if (t.variant == kDslashWaveSite) return launch_dslash_wave(a, t, stream);
+    if (t.variant == kDslashMfma) return launch_dslash_mfma(a, t, stream);
     switch (a.prec) {
         case kDouble: return dispatch_nc<double, double>(a, stream);
         case kSingle: return dispatch_nc<float, float>(a, stream);

@@ -1,0 +1,296 @@
+// dslash_mfma.hip -- Wilson hopping term on MFMA for FP16 / BF16 storage
+// (v_mfma_f32_16x16x32_{f16,bf16}, FP32 accumulation).
+//
+// One wavefront computes one output site.  For each output spin s the site's
+// result is a real GEMM
+//     D_s[rhs][(c, re|im)] = sum_k A_s[rhs][k] * B[k][(c, re|im)]
+// with k running over (hop h, input colour c', re|im): A_s holds the spin-
+// projected half spinor of the neighbour (h_r = psi_r + s psi_m(r) for the
+// upper spins, the reconstruction-scaled copy for the lower ones), B the real
+// 2x2 embedding of the link V_h (U_mu(x) forward, U_mu(x-mu)^+ backward):
+//     B[(c',re)][(c,re)] = Re V[c][c'],  B[(c',im)][(c,re)] = -Im V[c][c'],
+//     B[(c',re)][(c,im)] = Im V[c][c'],  B[(c',im)][(c,im)] =  Re V[c][c'].
+// Each hop takes 8 k entries (2 Nc <= 8, zero padded) or 16 (Nc = 8), so one
+// K = 32 instruction covers 4 hops (2 for Nc = 8): 8 MFMAs per site and 16
+// right-hand sides.  The instruction is issued transposed, D^T = B^T A^T
+// (links as the matrix-core A operand, spinors as B), so each lane's
+// accumulator rows are whole (re, im) pairs of its own RHS and go straight to
+// memory.  Lane l: operand row / column l % 16, k group l / 16; accumulator
+// rows 4*(l/16)..+3, column l % 16 (verified on gfx950).
+//
+// Every complex number is one 32-bit word (re in the low half), so the
+// spin projection and reconstruction are word operations: multiplying by
+// i^k is a 16-bit rotate plus a sign-bit XOR, the sum a packed add -- the
+// words go into the MFMA operands without any conversion.  Rows beyond R and
+// columns beyond Nc read clamped, valid addresses and are never stored, so
+// the kernel has no divergent branches around its loads.
+#include <cstdint>
+
+#include "dslash_common.h"
+#include "geom.h"
+
+namespace qcu {
+namespace {
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename S> struct Pk;
+template <> struct Pk<__half> {
+    __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b) {
+        return __builtin_bit_cast(uint32_t, __builtin_bit_cast(f16x2, a) + __builtin_bit_cast(f16x2, b));
+    }
+    __device__ __forceinline__ static float re(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[0]); }
+    __device__ __forceinline__ static float im(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[1]); }
+    __device__ __forceinline__ static uint32_t pack(float r, float i) {
+        const f16x2 v = {_Float16(r), _Float16(i)};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+    __device__ __forceinline__ static f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
+                                                      c, 0, 0, 0);
+    }
+};
+template <> struct Pk<__hip_bfloat16> {
+    __device__ __forceinline__ static float re(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+    __device__ __forceinline__ static float im(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+    __device__ __forceinline__ static uint32_t pack(float r, float i) {
+        const bf16x2 v = {__bf16(r), __bf16(i)};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+    __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b) {
+        return pack(re(a) + re(b), im(a) + im(b));
+    }
+    __device__ __forceinline__ static f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+};
+
+constexpr int kRawBufferFlags = 0x00020000;  // gfx9 raw buffer resource, dword 3
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const T *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(p), 0, int(bytes), kRawBufferFlags);
+}
+__device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+
+// v_perm_b32 byte selectors: identity and 16-bit rotate of a (re, im) word
+constexpr uint32_t kPermId = 0x03020100u, kPermRot = 0x01000302u;
+
+struct WordOp {  // z -> z * i^k on a packed word: byte permutation, then sign bits
+    uint32_t sel, mask;
+};
+__device__ __forceinline__ WordOp word_op(int k) {
+    // i^1: (-im, re)  i^2: (-re, -im)  i^3: (im, -re)
+    const uint32_t odd = uint32_t(k & 1), hi = uint32_t(k >> 1) & 1u;
+    WordOp w;
+    w.sel = kPermId ^ (odd * (kPermId ^ kPermRot));
+    w.mask = ((odd ^ hi) << 15) | (hi << 31);
+    return w;
+}
+__device__ __forceinline__ uint32_t apply(const WordOp &o, uint32_t w) {
+    return __builtin_amdgcn_perm(w, w, o.sel) ^ o.mask;
+}
+
+template <typename S, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
+                                                    const uint32_t *__restrict__ gauge,
+                                                    const uint32_t *__restrict__ xpay, float coef, int Vh, int R,
+                                                    int parity, const int *__restrict__ order,
+                                                    const int *__restrict__ nbrtab, int spw, unsigned vec_bytes,
+                                                    unsigned gauge_bytes) {
+    using P = Pk<S>;
+    constexpr int KH = (2 * NC <= 8) ? 8 : 16;  // k entries per hop
+    constexpr int HPS = 32 / KH;                 // hops per MFMA
+    constexpr int STEPS = 8 / HPS;               // K steps per site
+    constexpr int GPH = 4 / HPS;                 // lane groups per hop
+    constexpr int CPL = NC < 4 ? NC : 4;         // input colours per lane and hop
+
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15;
+    const int grp = lane >> 4;
+    const int csub = (grp % GPH) * 4;           // first input colour of this lane's k entries
+    const int cb = m >> 1;                      // output colour of B / D column m
+    const bool bre = (m & 1) == 0;
+    const bool bvalid = cb < NC;
+    const int cbc = bvalid ? cb : NC - 1;       // clamped (column never stored)
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const unsigned row_bytes = unsigned(R) * 4u;            // one (spin, colour) row of R words
+    const unsigned site_bytes = 4u * NC * row_bytes;
+    const unsigned ll = unsigned(NC) * NC;
+    const auto r_in = rsrc(in, vec_bytes);
+    const auto r_g = rsrc(gauge, gauge_bytes);
+    const auto r_out = rsrc(out, vec_bytes);
+    const auto r_x = rsrc(xpay, xpay ? vec_bytes : 0u);
+
+    // per-lane, per-step hop constants
+    int hop_s[STEPS], mu_s[STEPS];
+    bool fwd_s[STEPS], m02_s[STEPS];
+    WordOp p0_s[STEPS], p1_s[STEPS], r0_s[STEPS], r1_s[STEPS], bop_s[STEPS];
+#pragma unroll
+    for (int st = 0; st < STEPS; ++st) {
+        const int h = st * HPS + grp / GPH;
+        const int mu = h >> 1;
+        const bool fwd = (h & 1) == 0;
+        const HopInfo hi = hop_info(mu, fwd, DAG);
+        hop_s[st] = h;
+        mu_s[st] = mu;
+        fwd_s[st] = fwd;
+        m02_s[st] = hi.m0_is_2;
+        p0_s[st] = word_op(hi.kp0);
+        p1_s[st] = word_op(hi.kp1);
+        r0_s[st] = word_op(hi.kr0);
+        r1_s[st] = word_op(hi.kr1);
+        // B word from the stored link word U: forward V = U[cb][c'], backward
+        // V = conj(U[c'][cb]); real column (Vr, -Vi), imaginary column (Vi, Vr)
+        bop_s[st].sel = bre ? kPermId : kPermRot;
+        bop_s[st].mask = fwd ? (bre ? 0x80000000u : 0u) : (bre ? 0u : 0x00008000u);
+    }
+
+    for (int j = 0; j < spw; ++j) {
+        const int q = (b * spw + j) * 4 + wv;
+        if (q >= Vh) break;  // uniform
+        const int site = order[q];
+        const int *nrow = nbrtab + (long(parity) * Vh + site) * 8;
+
+        unsigned voff_a[STEPS], voff_b0[STEPS], voff_b1[STEPS];
+        u32x4 bfrag[STEPS];
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st) {
+            const int nb = nrow[hop_s[st]];
+            const unsigned va = unsigned(nb) * site_bytes;
+            // spin m(0) is 3 for x, y hops and 2 for z, t hops; spin 2 row is the base
+            voff_a[st] = va;
+            voff_b0[st] = va + (m02_s[st] ? 0u : unsigned(NC) * row_bytes);
+            voff_b1[st] = va + (m02_s[st] ? unsigned(NC) * row_bytes : 0u);
+            const unsigned gl = fwd_s[st] ? (unsigned(mu_s[st]) * 2 + parity) * unsigned(Vh) + unsigned(site)
+                                          : (unsigned(mu_s[st]) * 2 + 1 - parity) * unsigned(Vh) + unsigned(nb);
+            const unsigned gstride = fwd_s[st] ? 4u : 4u * NC;  // step of c' in bytes
+            const unsigned gbase = (gl * ll + unsigned(cbc) * (fwd_s[st] ? NC : 1)) * 4u + unsigned(csub) * gstride;
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                uint32_t w = 0u;
+                if (cc < CPL) w = apply(bop_s[st], bload(r_g, gbase + unsigned(cc) * gstride, 0u));
+                bfrag[st][cc] = w;
+            }
+        }
+
+        for (int r0 = 0; r0 < R; r0 += 16) {
+            const unsigned rcb = unsigned(min(r0 + m, R - 1)) * 4u;  // rows beyond R are never stored
+            f32x4 d[4];
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
+            // every neighbour word of this site first: independent loads
+            uint32_t w0[STEPS][4], w1[STEPS][4], wb0[STEPS][4], wb1[STEPS][4];
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st)
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    if (cc < CPL) {
+                        const unsigned c1 = unsigned(csub + cc) * row_bytes;
+                        w0[st][cc] = bload(r_in, voff_a[st] + rcb + c1, 0u);
+                        w1[st][cc] = bload(r_in, voff_a[st] + rcb + c1, unsigned(NC) * row_bytes);
+                        wb0[st][cc] = bload(r_in, voff_b0[st] + rcb + c1, 2u * NC * row_bytes);
+                        wb1[st][cc] = bload(r_in, voff_b1[st] + rcb + c1, 2u * NC * row_bytes);
+                    } else {
+                        w0[st][cc] = w1[st][cc] = wb0[st][cc] = wb1[st][cc] = 0u;
+                    }
+                }
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st) {
+                u32x4 a[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    const uint32_t h0 = P::add(w0[st][cc], apply(p0_s[st], wb0[st][cc]));
+                    const uint32_t h1 = P::add(w1[st][cc], apply(p1_s[st], wb1[st][cc]));
+                    const uint32_t g0 = apply(r0_s[st], h0), g1 = apply(r1_s[st], h1);
+                    a[0][cc] = h0;
+                    a[1][cc] = h1;
+                    a[2][cc] = m02_s[st] ? g0 : g1;
+                    a[3][cc] = m02_s[st] ? g1 : g0;
+                }
+                // links as the A operand, spinors as B: D^T = B^T A^T, so the
+                // accumulator rows are (colour, re|im) and its column is this lane's RHS
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp) d[sp] = P::mma(bfrag[st], a[sp], d[sp]);
+            }
+            // lane (m, grp) holds rows 4*grp + e = (colour 2*grp + e/2, re|im) of RHS r0 + m
+            const int r = r0 + m;
+            if (r < R) {
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp) {
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const int c = 2 * grp + i;
+                        if (c < NC) {
+                            const unsigned off = ((unsigned(site) * 4u + sp) * NC + unsigned(c)) * row_bytes + unsigned(r) * 4u;
+                            float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
+                            if (xpay) {
+                                const uint32_t xw = bload(r_x, off, 0u);
+                                vr += P::re(xw);
+                                vi += P::im(xw);
+                            }
+                            __builtin_amdgcn_raw_buffer_store_b32(int(P::pack(vr, vi)), r_out, int(off), 0, 0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename S, int NC>
+hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(a.g, t.by, t.bz, tab, stream));
+    dim3 grid(div_up(a.g.Vh, 4 * t.spw)), block(256);
+    auto out = static_cast<uint32_t *>(a.out);
+    auto in = static_cast<const uint32_t *>(a.in);
+    auto gauge = static_cast<const uint32_t *>(a.gauge);
+    auto xp = static_cast<const uint32_t *>(a.xpay);
+    const unsigned vec_bytes = unsigned(size_t(a.g.Vh) * 4 * a.Nc * a.R * 4);
+    const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
+    if (a.dagger)
+        hipLaunchKernelGGL((k_dslash_mfma<S, NC, true>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
+                           a.g.Vh, a.R, a.parity, tab.order, tab.nbr, t.spw, vec_bytes, gauge_bytes);
+    else
+        hipLaunchKernelGGL((k_dslash_mfma<S, NC, false>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
+                           a.g.Vh, a.R, a.parity, tab.order, tab.nbr, t.spw, vec_bytes, gauge_bytes);
+    return hipGetLastError();
+}
+
+template <typename S>
+hipError_t dispatch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch<S, 2>(a, t, stream);
+        case 3: return launch<S, 3>(a, t, stream);
+        case 4: return launch<S, 4>(a, t, stream);
+        case 8: return launch<S, 8>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool dslash_mfma_supported(int Nc, int prec) {
+    return (prec == kHalf || prec == kBF16) && (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8);
+}
+
+hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    if (!dslash_mfma_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
+    if (t.by <= 0 || t.bz <= 0 || t.spw <= 0 || a.g.L[1] % t.by || a.g.L[2] % t.bz) return hipErrorInvalidValue;
+    // buffer resources address at most 2^31 - 1 bytes
+    if (double(a.g.Vh) * 16.0 * a.Nc * a.R >= 2147483647.0 || double(a.g.Vh) * 32.0 * a.Nc * a.Nc >= 2147483647.0)
+        return hipErrorInvalidValue;
+    return a.prec == kHalf ? dispatch_nc<__half>(a, t, stream) : dispatch_nc<__hip_bfloat16>(a, t, stream);
+}
+
+}  // namespace qcu

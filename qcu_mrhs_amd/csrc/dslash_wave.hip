@@ -1,0 +1,178 @@
+// dslash_wave.hip -- wave-per-site Wilson hopping kernel (VALU).
+//
+// One wavefront computes one output site (looping over `spw` sites).  Its 64
+// lanes are 4 groups x 16 right-hand sides: group mu handles the two hops
+// along dimension mu (forward U_mu(x) psi(x+mu), backward U_mu(x-mu)^+ psi(x-mu))
+// for its RHS, so every lane loads two neighbour spinors and every load
+// instruction reads four contiguous 16-RHS rows.  The four partial sums are
+// then reduced with a transposing butterfly (lane xor 32, then xor 16) that
+// leaves output spin s in lane group s, so the store is one contiguous 16-RHS
+// row per (spin, colour).  Only a few sites per XCD are in flight, and the
+// traversal (tiles in y,z swept along t, contiguous per XCD) keeps every
+// neighbour reuse within the XCD's L2.
+#include "dslash_common.h"
+
+namespace qcu {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ Cx<T> shfl_xor_cx(Cx<T> v, int mask) {
+    return {__shfl_xor(v.re, mask), __shfl_xor(v.im, mask)};
+}
+
+template <typename T, typename CS, int NC, bool FWD>
+__device__ __forceinline__ void hop_rt(Cx<T> (&acc)[4][NC], const CS *__restrict__ psi, int R,
+                                       const CS *__restrict__ U, const HopInfo &h) {
+    Cx<T> h0[NC], h1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const Cx<T> a0 = load_cx<T>(psi + (0 * NC + c) * R);
+        const Cx<T> a1 = load_cx<T>(psi + (1 * NC + c) * R);
+        const Cx<T> a2 = load_cx<T>(psi + (2 * NC + c) * R);
+        const Cx<T> a3 = load_cx<T>(psi + (3 * NC + c) * R);
+        const Cx<T> b0 = h.m0_is_2 ? a2 : a3;
+        const Cx<T> b1 = h.m0_is_2 ? a3 : a2;
+        h0[c] = a0 + ipow_rt(b0, h.kp0);
+        h1[c] = a1 + ipow_rt(b1, h.kp1);
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        Cx<T> s0 = {T(0), T(0)}, s1 = {T(0), T(0)};
+#pragma unroll
+        for (int b = 0; b < NC; ++b) {
+            if constexpr (FWD) {
+                const Cx<T> u = load_cx<T>(U + a * NC + b);
+                cfma(s0, u, h0[b]);
+                cfma(s1, u, h1[b]);
+            } else {
+                const Cx<T> u = load_cx<T>(U + b * NC + a);  // (U^+)[a][b] = conj(U[b][a])
+                cfma_conj(s0, u, h0[b]);
+                cfma_conj(s1, u, h1[b]);
+            }
+        }
+        acc[0][a] = acc[0][a] + s0;
+        acc[1][a] = acc[1][a] + s1;
+        const Cx<T> r0 = ipow_rt(s0, h.kr0), r1 = ipow_rt(s1, h.kr1);
+        acc[2][a] = acc[2][a] + (h.m0_is_2 ? r0 : r1);
+        acc[3][a] = acc[3][a] + (h.m0_is_2 ? r1 : r0);
+    }
+}
+
+template <typename T, typename CS, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_wave(CS *__restrict__ out, const CS *__restrict__ in,
+                                                    const CS *__restrict__ gauge,
+                                                    const CS *__restrict__ xpay, T coef, Geom g, int R,
+                                                    int parity, int by, int bz, int spw) {
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int mu = lane >> 4;
+    const int rl = lane & 15;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const long site_len = 4L * NC * R;
+    const long link_len = long(NC) * NC;
+    const long Vh = g.Vh;
+    const HopInfo hf = hop_info(mu, true, DAG), hb = hop_info(mu, false, DAG);
+    const NbConst ncf = nb_const(g, mu, true), ncb = nb_const(g, mu, false);
+    const bool lo = lane < 32, odd = (mu & 1) != 0;
+
+    for (int j = 0; j < spw; ++j) {
+        const int q = (b * spw + j) * 4 + wv;
+        if (q >= g.Vh) break;  // uniform within the wavefront
+        const int site = traverse(q, g, by, bz);
+        const Site s = site_coords(site, g, parity);
+        const int nf = neighbour_bl(site, s, g, ncf);
+        const int nb = neighbour_bl(site, s, g, ncb);
+        const CS *Uf = gauge + ((mu * 2 + parity) * Vh + site) * link_len;
+        const CS *Ub = gauge + ((mu * 2 + 1 - parity) * Vh + nb) * link_len;
+        for (int r0 = 0; r0 < R; r0 += 16) {
+            const int r = r0 + rl;
+            const bool act = r < R;
+            Cx<T> acc[4][NC];
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) acc[sp][c] = {T(0), T(0)};
+            if (act) {
+                hop_rt<T, CS, NC, true>(acc, in + nf * site_len + r, R, Uf, hf);
+                hop_rt<T, CS, NC, false>(acc, in + nb * site_len + r, R, Ub, hb);
+            }
+            // lanes < 32 keep spins {0,1}, lanes >= 32 keep spins {2,3}
+            Cx<T> kept[2][NC];
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const Cx<T> send = lo ? acc[2 + s2][c] : acc[s2][c];
+                    const Cx<T> recv = shfl_xor_cx(send, 32);
+                    kept[s2][c] = (lo ? acc[s2][c] : acc[2 + s2][c]) + recv;
+                }
+            // lane group mu keeps spin mu
+            Cx<T> fin[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const Cx<T> send = odd ? kept[0][c] : kept[1][c];
+                const Cx<T> recv = shfl_xor_cx(send, 16);
+                fin[c] = (odd ? kept[1][c] : kept[0][c]) + recv;
+            }
+            if (act) {
+                const long base = (site * 4L + mu) * NC * R + r;
+                if (xpay) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        Cx<T> v = load_cx<T>(xpay + base + c * R);
+                        v.re = fma(coef, fin[c].re, v.re);
+                        v.im = fma(coef, fin[c].im, v.im);
+                        store_cx(out + base + c * R, v);
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) store_cx(out + base + c * R, coef * fin[c]);
+                }
+            }
+        }
+    }
+}
+
+template <typename T, typename S, int NC>
+hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    using CS = typename CxStore<S>::type;
+    const int sites_per_block = 4 * t.spw;
+    dim3 grid(div_up(a.g.Vh, sites_per_block)), block(256);
+    auto out = static_cast<CS *>(a.out);
+    auto in = static_cast<const CS *>(a.in);
+    auto gauge = static_cast<const CS *>(a.gauge);
+    auto xp = static_cast<const CS *>(a.xpay);
+    if (a.dagger)
+        hipLaunchKernelGGL((k_dslash_wave<T, CS, NC, true>), grid, block, 0, stream, out, in, gauge, xp,
+                           T(a.coef), a.g, a.R, a.parity, t.by, t.bz, t.spw);
+    else
+        hipLaunchKernelGGL((k_dslash_wave<T, CS, NC, false>), grid, block, 0, stream, out, in, gauge, xp,
+                           T(a.coef), a.g, a.R, a.parity, t.by, t.bz, t.spw);
+    return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t dispatch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch<T, S, 2>(a, t, stream);
+        case 3: return launch<T, S, 3>(a, t, stream);
+        case 4: return launch<T, S, 4>(a, t, stream);
+        case 8: return launch<T, S, 8>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    if (t.by <= 0 || t.bz <= 0 || t.spw <= 0 || a.g.L[1] % t.by || a.g.L[2] % t.bz) return hipErrorInvalidValue;
+    switch (a.prec) {
+        case kDouble: return dispatch_nc<double, double>(a, t, stream);
+        case kSingle: return dispatch_nc<float, float>(a, t, stream);
+        case kHalf: return dispatch_nc<float, __half>(a, t, stream);
+        case kBF16: return dispatch_nc<float, __hip_bfloat16>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace qcu

@@ -482,6 +482,35 @@ int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *
     return hip_ok(launch_scatter(t, dst_precision, src, src_precision, nsites * 4 * n_color, m_rhs, nullptr),
                   "scatter") ? 0 : g_err;
 }
+int qcuDslashTune(int variant, int by, int bz, int spw) {
+    if (variant < kDslashAuto || variant > kDslashMfma || by < 0 || bz < 0 || spw < 0) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashTune: invalid arguments");
+        return g_err;
+    }
+    DslashTune t;
+    t.variant = variant;
+    t.by = by;
+    t.bz = bz;
+    t.spw = spw;
+    dslash_set_override(t);
+    return 0;
+}
+
+static hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+int qcuTimerStart(void) {
+    if (!g_ev0 && (!hip_ok(hipEventCreate(&g_ev0), "event") || !hip_ok(hipEventCreate(&g_ev1), "event")))
+        return g_err;
+    return hip_ok(hipEventRecord(g_ev0, nullptr), "event") ? 0 : g_err;
+}
+double qcuTimerStopMs(void) {
+    if (!g_ev0) return -1.0;
+    float ms = -1.0f;
+    if (!hip_ok(hipEventRecord(g_ev1, nullptr), "event") || !hip_ok(hipEventSynchronize(g_ev1), "event") ||
+        !hip_ok(hipEventElapsedTime(&ms, g_ev0, g_ev1), "event"))
+        return -1.0;
+    return double(ms);
+}
+
 int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_precision, long count) {
     if (!valid_storage(dst_precision) || !valid_storage(src_precision) || count < 0) {
         set_error(kQcuErrorInvalidValue, "copy: invalid arguments");

@@ -132,15 +132,18 @@ template <typename T>
 __device__ __forceinline__ Cx<T> load_cx(const Cx<double> *p) { Cx<double> v = *p; return {T(v.re), T(v.im)}; }
 template <typename T>
 __device__ __forceinline__ Cx<T> load_cx(const Cx<float> *p) { Cx<float> v = *p; return {T(v.re), T(v.im)}; }
+typedef _Float16 qcu_f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 qcu_bf16x2 __attribute__((ext_vector_type(2)));
+// 16-bit pairs travel as one 32-bit word (re in the low half)
 template <typename T>
 __device__ __forceinline__ Cx<T> load_cx(const Half2Store *p) {
-    Half2Store v = *p;
-    return {T(__half2float(v.re)), T(__half2float(v.im))};
+    const qcu_f16x2 v = __builtin_bit_cast(qcu_f16x2, *reinterpret_cast<const uint32_t *>(p));
+    return {T(float(v[0])), T(float(v[1]))};
 }
 template <typename T>
 __device__ __forceinline__ Cx<T> load_cx(const Bf162Store *p) {
-    Bf162Store v = *p;
-    return {T(__bfloat162float(v.re)), T(__bfloat162float(v.im))};
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+    return {T(__builtin_bit_cast(float, w << 16)), T(__builtin_bit_cast(float, w & 0xffff0000u))};
 }
 template <typename T>
 __device__ __forceinline__ void store_cx(Cx<double> *p, Cx<T> v) { *p = Cx<double>{double(v.re), double(v.im)}; }
@@ -148,17 +151,13 @@ template <typename T>
 __device__ __forceinline__ void store_cx(Cx<float> *p, Cx<T> v) { *p = Cx<float>{float(v.re), float(v.im)}; }
 template <typename T>
 __device__ __forceinline__ void store_cx(Half2Store *p, Cx<T> v) {
-    Half2Store h;
-    h.re = __float2half(float(v.re));
-    h.im = __float2half(float(v.im));
-    *p = h;
+    const qcu_f16x2 h = {_Float16(float(v.re)), _Float16(float(v.im))};
+    *reinterpret_cast<uint32_t *>(p) = __builtin_bit_cast(uint32_t, h);
 }
 template <typename T>
 __device__ __forceinline__ void store_cx(Bf162Store *p, Cx<T> v) {
-    Bf162Store h;
-    h.re = __float2bfloat16(float(v.re));
-    h.im = __float2bfloat16(float(v.im));
-    *p = h;
+    const qcu_bf16x2 h = {__bf16(float(v.re)), __bf16(float(v.im))};
+    *reinterpret_cast<uint32_t *>(p) = __builtin_bit_cast(uint32_t, h);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,71 @@
+"""GPU parity of every dslash kernel variant (lane-per-RHS, wave-per-site
+VALU, wave-per-site MFMA) and launch shape against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from qcu_mrhs_amd import api, fields
+
+pytestmark = pytest.mark.gpu
+
+D, S, H, B = 2, 1, 0, 4
+TOL = {D: 1e-13, S: 2e-7, H: 2e-3, B: 1.5e-2}
+
+SHAPES = [((8, 8, 8, 8), 3, 4), ((4, 4, 4, 4), 3, 16), ((4, 2, 4, 6), 3, 12), ((4, 4, 4, 8), 2, 3),
+          ((4, 4, 4, 4), 4, 5), ((4, 4, 2, 4), 8, 2), ((6, 4, 4, 4), 3, 17), ((4, 4, 4, 4), 3, 32)]
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(autouse=True)
+def _reset_tune():
+    yield
+    api.dslash_tune("auto")
+
+
+@pytest.mark.parametrize("variant", ["lane_rhs", "wave", "mfma"])
+@pytest.mark.parametrize("L,Nc,R", SHAPES)
+def test_variant_matches_oracle(variant, L, Nc, R):
+    print("case", variant, L, Nc, R, flush=True)
+    precs = [H, B] if variant == "mfma" else [D, S, H, B]
+    gauge = fields.gauge_field(L, Nc, 300 + Nc + R, kind="haar")
+    src = fields.gaussian_sources(L, Nc, R, 400 + R).reshape(R, 2, -1)
+    api.dslash_tune(variant)
+    for prec in precs:
+        d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+        for parity, dagger in ((0, 0), (1, 1), (1, 0)):
+            inp = fields.to_mrhs(src[:, 1 - parity], Nc)
+            xp = fields.to_mrhs(src[:, parity], Nc)
+            d_in = api.DeviceArray.from_complex(inp, prec)
+            d_xp = api.DeviceArray.from_complex(xp, prec)
+            d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec)
+            ref = oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)
+            got = d_out.complex(inp.size, prec)
+            assert _rel(got, ref) < TOL[prec], (prec, parity, dagger, _rel(got, ref))
+            # fused epilogue out = x - 0.13 D in
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
+            got = d_out.complex(inp.size, prec)
+            assert _rel(got, xp - 0.13 * ref) < TOL[prec], (prec, "xpay")
+
+
+@pytest.mark.parametrize("variant", ["wave", "mfma"])
+@pytest.mark.parametrize("by,bz,spw", [(1, 1, 1), (2, 4, 3), (4, 2, 5), (8, 8, 2)])
+def test_traversal_tiles_and_sites_per_wave(variant, by, bz, spw):
+    """Every (y, z) tiling and sites-per-wave count covers each site exactly once."""
+    L, Nc, R = (6, 8, 8, 6), 3, 8
+    prec = H if variant == "mfma" else D
+    gauge = fields.gauge_field(L, Nc, 7, kind="haar")
+    inp = fields.to_mrhs(fields.gaussian_sources(L, Nc, R, 8).reshape(R, 2, -1)[:, 0], Nc)
+    api.dslash_tune(variant, by, bz, spw)
+    d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+    d_in = api.DeviceArray.from_complex(inp, prec)
+    d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+    d_out.zero()
+    api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, 1, 0, precision=prec)
+    ref = oracle.dslash(inp, gauge, L, Nc, R, 1, 0)
+    got = d_out.complex(inp.size, prec)
+    per_site = np.abs(got - ref).reshape(-1, 4 * Nc * R).max(axis=1) / np.abs(ref).max()
+    assert per_site.max() < TOL[prec] * 10, np.argmax(per_site)
