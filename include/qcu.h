@@ -126,6 +126,33 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info);
  * use_preconditioner: 1 = apply the loaded learned right preconditioner. */
 int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditioner);
 
+/* ---- learned preconditioner (no reference counterpart) ---------------
+ * A linear two-layer gauge-covariant network (parallel-transport
+ * convolutions, 1 -> C -> 1 channels, nearest-neighbour hops, complex 4x4
+ * spin-matrix weights) used as a fixed right preconditioner of the odd
+ * Schur system when qcuSetSolverOptions(..., use_preconditioner = 1).
+ * Weights: W1[C][9][4][4], W2[C][9][4][4] complex, passed as interleaved
+ * (re, im) doubles; hop 0 = identity, hop 1 + 2 mu (+1) = forward (backward)
+ * transport along mu.  All calls use the gauge field loaded by loadQcuGauge. */
+int qcuLgnnSetWeights(int channels, const double *w1, const double *w2);
+int qcuLgnnGetWeights(double *w1, double *w2);
+/* out = N in on odd-parity multi-RHS half vectors [Vh][4][Nc][R] (device). */
+int qcuLgnnApply(void *out, const void *in, int precision);
+/* One forward/backward pass on a batch of R pairs (FP32 complex odd half
+ * vectors): loss = sum_r ||N in_r - target_r||^2 / ||target_r||^2 and the
+ * gradients dL/dRe W + i dL/dIm W (host arrays shaped like the weights). */
+int qcuLgnnLossGrad(const void *in, const void *target, double *loss, double *gw1, double *gw2);
+/* Solve the queued systems with plain FP64 BiCGStab and keep n_snapshots
+ * (error, residual) pairs per RHS -- e_k = x - x_k, r_k = A e_k of the odd
+ * Schur system at evenly spaced iterations -- as training data for the
+ * current gauge field.  *n_batches receives the total number of stored
+ * batches (R pairs each).  The queue is consumed. */
+int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batches);
+/* Adam over the stored batches (one step per batch, order shuffled with
+ * `seed`); loss_per_epoch[e] = mean loss per pair. */
+int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_per_epoch);
+int qcuLgnnClearData(void);
+
 /* Device-memory plumbing for callers that have no other GPU runtime. */
 void *qcuDeviceMalloc(size_t bytes);
 int qcuDeviceFree(void *ptr);

@@ -67,6 +67,18 @@ int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_
  * sites per wavefront (0 = choose).  No reference counterpart. */
 int qcuDslashTune(int variant, int by, int bz, int spw);
 
+/* Autotune the dslash for (lattice, n_color, m_rhs, precision): times every
+ * kernel variant over sites-per-wavefront and traversal-tile candidates and
+ * records the fastest for later automatic launches.  best[4] receives
+ * {variant, by, bz, spw}, *best_us its kernel time; records (may be NULL,
+ * capacity max_records, 5 doubles each: variant, by, bz, spw, us) and
+ * *n_records list every candidate.  No reference counterpart. */
+int qcuDslashAutotune(const int lattice[4], int n_color, int m_rhs, int precision, int repeats, int best[4],
+                      double *best_us, double *records, int max_records, int *n_records);
+/* Enter a launch shape into the tuning table (e.g. from a saved tuning cache). */
+int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
+                      int spw);
+
 /* Device timing of the stateless operations: HIP events recorded on the
  * stream those operations run on.  qcuTimerStopMs waits for the stop event
  * and returns the elapsed milliseconds (negative on error). */

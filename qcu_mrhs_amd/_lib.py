@@ -83,6 +83,17 @@ def load() -> ctypes.CDLL:
         "qcuColorSpinorScatter": ([ctypes.POINTER(vp), i, vp, i, l, i, i], i),
         "qcuCopyComplexVector": ([vp, i, vp, i, l], i),
         "qcuDslashTune": ([i, i, i, i], i),
+        "qcuDslashAutotune": ([ip4, i, i, i, i, ip4, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), i, ip4], i),
+        "qcuDslashSetTuned": ([ip4, i, i, i, i, i, i, i], i),
+        "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
+        "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
+        "qcuLgnnApply": ([vp, vp, i], i),
+        "qcuLgnnLossGrad": ([vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double)], i),
+        "qcuLgnnCollect": ([i, d, i, ip4], i),
+        "qcuLgnnTrain": ([i, d, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)], i),
+        "qcuLgnnClearData": ([], i),
         "qcuTimerStart": ([], i),
         "qcuTimerStopMs": ([], d),
     }

@@ -1,18 +1,83 @@
-// lgnn.h -- learned gauge-covariant right preconditioner (linear two-layer
-// parallel-transport network) for the odd Schur system.
+// lgnn.h -- learned gauge-covariant right preconditioner for the odd Schur
+// system: two linear parallel-transport convolution layers 1 -> C -> 1
+// (C = 8 channels, nearest-neighbour hops, complex spin-matrix weights).
+//
+// Because the layers are linear and their weights act on spin only, the
+// network restricted to odd sites equals the two-hop stencil
+//     N psi = M0 psi + sum_{h,h'=1..8} T_h M[h][h'] T_h' psi,
+//     M0 = sum_o W2[o][0] W1[o][0],   M[h][h'] = sum_o W2[o][h] W1[o][h'],
+// with T_h the parallel transport along hop h (forward U_mu(x) psi(x+mu),
+// backward U_mu(x-mu)^+ psi(x-mu)).  The GPU evaluates it in three steps:
+// transport of the odd input to the eight hop fields on the even sites, the
+// (hop, spin) mixing by the 32 x 32 complex matrix M, and the transported
+// gather back onto the odd sites.  Gradients with respect to M0 and M are
+// outer-product reductions; the chain rule to W1, W2 and the Adam update run
+// on the device as well.
 #pragma once
 
+#include <vector>
+
+#include "blas.h"
 #include "solver.h"
 
 namespace qcu {
 
+constexpr int kLgnnHops = 9;
+
 class LgnnPreconditioner : public Preconditioner {
 public:
-    LgnnPreconditioner() = default;
-    ~LgnnPreconditioner() override = default;
-    bool ready() const { return false; }
-    bool supports(int) const override { return false; }
-    hipError_t apply(void *, const void *, int, hipStream_t) override { return hipErrorNotSupported; }
+    LgnnPreconditioner(const Geom &g, int Nc, int R, int channels);
+    ~LgnnPreconditioner() override;
+    LgnnPreconditioner(const LgnnPreconditioner &) = delete;
+    LgnnPreconditioner &operator=(const LgnnPreconditioner &) = delete;
+
+    bool ready() const { return have_weights_ && gauge_ != nullptr; }
+    bool supports(int prec) const override;
+    int channels() const { return C_; }
+
+    // Gauge field the transports use (FP64 even/odd layout, device); a
+    // single-precision copy is kept.  Must be set before apply().
+    hipError_t set_gauge(const void *gauge64, hipStream_t s);
+    // Weights: W1 [C][9][4][4], W2 [C][9][4][4] complex FP64 (host arrays).
+    hipError_t set_weights(const double *w1, const double *w2, hipStream_t s);
+    hipError_t get_weights(double *w1, double *w2, hipStream_t s);
+
+    // out = N in on odd-parity MRHS half vectors stored in precision `prec`
+    hipError_t apply(void *out, const void *in, int prec, hipStream_t s) override;
+
+    // One forward/backward pass on a batch of R pairs (FP32 odd MRHS vectors):
+    // loss = sum_r ||N r_r - e_r||^2 / ||e_r||^2; gradients dL/dRe W + i dL/dIm W
+    // accumulated into the device gradient buffers (zeroed by zero_grad()).
+    hipError_t zero_grad(hipStream_t s);
+    hipError_t loss_grad(const void *in, const void *target, double *loss_host, hipStream_t s);
+    hipError_t read_grad(double *gw1, double *gw2, hipStream_t s);
+    // Adam step with the accumulated gradient (divided by `batches`).
+    hipError_t adam_step(double lr, int batches, hipStream_t s);
+
+private:
+    hipError_t ensure_fields(hipStream_t s);
+    hipError_t collapse(hipStream_t s);
+    hipError_t forward(const void *in, int prec, void *out, int out_prec, hipStream_t s);
+
+    Geom g_;
+    int Nc_, R_, C_;
+    long n_;                        // complex entries per RHS of a half vector
+    bool have_weights_ = false;
+    const void *gauge_ = nullptr;   // FP64 gauge (borrowed)
+    void *gauge32_ = nullptr;       // FP32 copy
+    // parameters (FP64 complex) and Adam moments
+    C64 *w1_ = nullptr, *w2_ = nullptr;          // [C][9][16]
+    C64 *g1_ = nullptr, *g2_ = nullptr;          // gradients
+    C64 *m1_ = nullptr, *m2_ = nullptr, *v1_ = nullptr, *v2_ = nullptr;  // Adam (v stores |g|^2 in re)
+    int adam_t_ = 0;
+    // collapsed stencil (FP32 complex): M0 [16], M [64][16] as a 32 x 32 matrix
+    Cx<float> *mcol_ = nullptr;                  // [32*32 + 16]
+    // work fields (FP32): u[8] hop fields on even sites, q[8], odd scratch
+    void *u_ = nullptr, *q_ = nullptr, *inf_ = nullptr, *phi_ = nullptr, *dlt_ = nullptr;
+    C64 *gm_ = nullptr;                          // dL/dM* partials [nblk][32*32 + 16]
+    int gram_blocks_ = 0;
+    double *dnorm_ = nullptr, *dloss_ = nullptr, *hbuf_ = nullptr;
+    ReduceWork red_;
 };
 
 }  // namespace qcu

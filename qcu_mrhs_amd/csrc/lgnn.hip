@@ -1,0 +1,574 @@
+// lgnn.hip -- GPU kernels and training of the learned preconditioner (lgnn.h).
+//
+// Hop numbering: k = 2 mu + d (d = 0 forward, 1 backward) = the column of the
+// neighbour table; the network's hop index is h = k + 1 (h = 0 is the
+// identity).  Fields are RHS-innermost half vectors [Vh][spin 4][Nc][R];
+// the eight hop fields of one half vector are stored back to back.
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "geom.h"
+#include "lgnn.h"
+
+namespace qcu {
+namespace {
+
+using C32 = Cx<float>;
+
+// out_k(y) = V_k(y) in(nb_k(y)) for every site y of parity `pout` and hop k;
+// V = U_mu(y) forward, U_mu(y - mu)^+ backward (link on the neighbour parity).
+template <typename CS, int NC>
+__global__ __launch_bounds__(256) void k_transport8(C32 *__restrict__ out, const CS *__restrict__ in,
+                                                   const C32 *__restrict__ gauge, const int *__restrict__ nbr,
+                                                   int Vh, int R, int pout) {
+    const int site = blockIdx.x * blockDim.y + threadIdx.y;
+    if (site >= Vh) return;
+    const int r = threadIdx.x;
+    const long sl = 4L * NC * R;
+    const long fsize = long(Vh) * sl;
+    const int *nrow = nbr + (long(pout) * Vh + site) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int mu = k >> 1;
+        const bool fwd = (k & 1) == 0;
+        const int nb = nrow[k];
+        const C32 *U = fwd ? gauge + ((long(mu) * 2 + pout) * Vh + site) * NC * NC
+                           : gauge + ((long(mu) * 2 + 1 - pout) * Vh + nb) * NC * NC;
+        const CS *src = in + nb * sl + r;
+        C32 *dst = out + k * fsize + site * sl + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            C32 v[NC];
+#pragma unroll
+            for (int b = 0; b < NC; ++b) v[b] = load_cx<float>(src + (s * NC + b) * R);
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                C32 acc = {0.f, 0.f};
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    if (fwd) cfma(acc, U[a * NC + b], v[b]);
+                    else cfma_conj(acc, U[b * NC + a], v[b]);
+                }
+                dst[(s * NC + a) * R] = acc;
+            }
+        }
+    }
+}
+
+// q[(k,s)] = sum_{(k',s')} M[(k,s)][(k',s')] u[(k',s')] per (even site, colour, rhs)
+__global__ __launch_bounds__(256) void k_mix(C32 *__restrict__ q, const C32 *__restrict__ u,
+                                            const C32 *__restrict__ mcol, long ncol, int Nc, int R, long fsize) {
+    __shared__ C32 M[32 * 32];
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) M[i] = mcol[i];
+    __syncthreads();
+    const long stride = long(gridDim.x) * blockDim.x;
+    for (long j = long(blockIdx.x) * blockDim.x + threadIdx.x; j < ncol; j += stride) {
+        // j enumerates (site, colour, rhs) with rhs fastest
+        const long r = j % R;
+        const long rest = j / R;
+        const long c = rest % Nc;
+        const long site = rest / Nc;
+        C32 x[32];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) x[k * 4 + s] = u[k * fsize + ((site * 4 + s) * Nc + c) * R + r];
+#pragma unroll 4
+        for (int i = 0; i < 32; ++i) {
+            C32 acc = {0.f, 0.f};
+#pragma unroll
+            for (int jj = 0; jj < 32; ++jj) cfma(acc, M[i * 32 + jj], x[jj]);
+            q[(i >> 2) * fsize + ((site * 4 + (i & 3)) * Nc + c) * R + r] = acc;
+        }
+    }
+}
+
+// out(x) = M0 in(x) + sum_k V_k(x) q_k(nb_k(x)) on the odd sites
+template <typename CS, typename CO, int NC>
+__global__ __launch_bounds__(256) void k_gather8(CO *__restrict__ out, const CS *__restrict__ in,
+                                                const C32 *__restrict__ q, const C32 *__restrict__ gauge,
+                                                const int *__restrict__ nbr, const C32 *__restrict__ m0, int Vh,
+                                                int R) {
+    const int site = blockIdx.x * blockDim.y + threadIdx.y;
+    if (site >= Vh) return;
+    const int r = threadIdx.x;
+    const long sl = 4L * NC * R;
+    const long fsize = long(Vh) * sl;
+    const int pout = 1;
+    C32 acc[4][NC];
+    {
+        C32 v[4][NC];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[s][c] = load_cx<float>(in + site * sl + (s * NC + c) * R + r);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                C32 a = {0.f, 0.f};
+#pragma unroll
+                for (int t = 0; t < 4; ++t) cfma(a, m0[s * 4 + t], v[t][c]);
+                acc[s][c] = a;
+            }
+    }
+    const int *nrow = nbr + (long(pout) * Vh + site) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int mu = k >> 1;
+        const bool fwd = (k & 1) == 0;
+        const int nb = nrow[k];
+        const C32 *U = fwd ? gauge + ((long(mu) * 2 + pout) * Vh + site) * NC * NC
+                           : gauge + ((long(mu) * 2 + 1 - pout) * Vh + nb) * NC * NC;
+        const C32 *src = q + k * fsize + nb * sl + r;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            C32 v[NC];
+#pragma unroll
+            for (int b = 0; b < NC; ++b) v[b] = src[(s * NC + b) * R];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    if (fwd) cfma(acc[s][a], U[a * NC + b], v[b]);
+                    else cfma_conj(acc[s][a], U[b * NC + a], v[b]);
+                }
+            }
+        }
+    }
+    CO *o = out + site * sl + r;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) store_cx(o + (s * NC + c) * R, acc[s][c]);
+}
+
+// Partial sums of G[(k,s)][(k',s')] = sum_j a_{k^1}[s](j) conj(b_k'[s'](j)) over
+// columns j = (even site, colour, rhs); a = transported dL/dphi*, b = u.
+// Block partials in FP64; thread t owns entries 4t .. 4t+3.
+constexpr int kGramTile = 32;
+__global__ __launch_bounds__(256) void k_gram(C64 *__restrict__ part, const C32 *__restrict__ a,
+                                             const C32 *__restrict__ b, long ncol, int Nc, int R, long fsize) {
+    __shared__ C32 sa[32][kGramTile];
+    __shared__ C32 sb[32][kGramTile];
+    const int t = threadIdx.x;
+    const int i = t >> 3;            // row (k, s)
+    const int j0 = (t & 7) * 4;      // first of 4 columns (k', s')
+    C64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const long per_block = (ncol + gridDim.x - 1) / gridDim.x;
+    const long lo = per_block * blockIdx.x;
+    const long hi = lo + per_block < ncol ? lo + per_block : ncol;
+    for (long base = lo; base < hi; base += kGramTile) {
+        // stage kGramTile columns of both 32-vectors
+        for (int e = t; e < 32 * kGramTile; e += blockDim.x) {
+            const int row = e / kGramTile, col = e % kGramTile;
+            const long j = base + col;
+            C32 va = {0.f, 0.f}, vb = {0.f, 0.f};
+            if (j < hi) {
+                const long r = j % R;
+                const long rest = j / R;
+                const long c = rest % Nc;
+                const long site = rest / Nc;
+                const int k = row >> 2, s = row & 3;
+                const long off = ((site * 4 + s) * Nc + c) * R + r;
+                va = a[(k ^ 1) * fsize + off];
+                vb = b[k * fsize + off];
+            }
+            sa[row][col] = va;
+            sb[row][col] = vb;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int col = 0; col < kGramTile; ++col) {
+            const C32 x = sa[i][col];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const C32 y = sb[j0 + q][col];
+                acc[q].re += double(x.re) * y.re + double(x.im) * y.im;
+                acc[q].im += double(x.im) * y.re - double(x.re) * y.im;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part[long(blockIdx.x) * 1040 + i * 32 + j0 + q] = acc[q];
+}
+
+// Partial sums of G0[s][t] = sum_j d[s](j) conj(p[t](j)) over odd-site columns.
+__global__ __launch_bounds__(256) void k_gram0(C64 *__restrict__ part, const C32 *__restrict__ d,
+                                              const C32 *__restrict__ p, long ncol, int Nc, int R) {
+    __shared__ C64 red[16][256];
+    C64 acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = {0.0, 0.0};
+    const long stride = long(gridDim.x) * blockDim.x;
+    for (long j = long(blockIdx.x) * blockDim.x + threadIdx.x; j < ncol; j += stride) {
+        const long r = j % R;
+        const long rest = j / R;
+        const long c = rest % Nc;
+        const long site = rest / Nc;
+        C32 dv[4], pv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const long off = ((site * 4 + s) * Nc + c) * R + r;
+            dv[s] = d[off];
+            pv[s] = p[off];
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t2 = 0; t2 < 4; ++t2) {
+                acc[s * 4 + t2].re += double(dv[s].re) * pv[t2].re + double(dv[s].im) * pv[t2].im;
+                acc[s * 4 + t2].im += double(dv[s].im) * pv[t2].re - double(dv[s].re) * pv[t2].im;
+            }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) red[e][threadIdx.x] = acc[e];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        C64 s = {0.0, 0.0};
+        for (int k = 0; k < int(blockDim.x); ++k) {
+            s.re += red[threadIdx.x][k].re;
+            s.im += red[threadIdx.x][k].im;
+        }
+        part[long(blockIdx.x) * 1040 + 1024 + threadIdx.x] = s;
+    }
+}
+
+// Sum block partials (fixed order) and apply the chain rule to W1, W2:
+//   dL/dM* = G, dL/dW2[o][h]* = sum_h' G[h][h'] W1[o][h']^+, dL/dW1[o][h']* = sum_h W2[o][h]^+ G[h][h']
+// (h = 0 pairs with M0).  The stored gradient is 2 dL/dW* = dL/dRe W + i dL/dIm W,
+// accumulated over batches.
+__global__ void k_chain(C64 *g1, C64 *g2, const C64 *w1, const C64 *w2, const C64 *part, int nblk, int C) {
+    __shared__ C64 G[1040];
+    for (int e = threadIdx.x; e < 1040; e += blockDim.x) {
+        C64 s = {0.0, 0.0};
+        for (int b = 0; b < nblk; ++b) {
+            s.re += part[long(b) * 1040 + e].re;
+            s.im += part[long(b) * 1040 + e].im;
+        }
+        G[e] = s;
+    }
+    __syncthreads();
+    // entry (o, h, s, t) of W: flattened ((o * 9 + h) * 4 + s) * 4 + t
+    const int total = C * 9 * 16;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int t = e & 3, s = (e >> 2) & 3, h = (e >> 4) % 9, o = e / 144;
+        C64 a2 = {0.0, 0.0}, a1 = {0.0, 0.0};
+        if (h == 0) {
+            // W2[o][0]: sum_u G0[s][u] conj(W1[o][0][t][u]);  W1[o][0]: sum_u conj(W2[o][0][u][s]) G0[u][t]
+            for (int u = 0; u < 4; ++u) {
+                const C64 gsu = G[1024 + s * 4 + u];
+                const C64 w1tu = w1[(o * 9) * 16 + t * 4 + u];
+                a2.re += gsu.re * w1tu.re + gsu.im * w1tu.im;
+                a2.im += gsu.im * w1tu.re - gsu.re * w1tu.im;
+                const C64 w2us = w2[(o * 9) * 16 + u * 4 + s];
+                const C64 gut = G[1024 + u * 4 + t];
+                a1.re += w2us.re * gut.re + w2us.im * gut.im;
+                a1.im += w2us.re * gut.im - w2us.im * gut.re;
+            }
+        } else {
+            const int k = h - 1;
+            for (int kp = 0; kp < 8; ++kp)
+                for (int u = 0; u < 4; ++u) {
+                    // W2[o][h][s][t] += G[(k,s)][(kp,u)] conj(W1[o][kp+1][t][u])
+                    const C64 gg = G[(k * 4 + s) * 32 + kp * 4 + u];
+                    const C64 w = w1[(o * 9 + kp + 1) * 16 + t * 4 + u];
+                    a2.re += gg.re * w.re + gg.im * w.im;
+                    a2.im += gg.im * w.re - gg.re * w.im;
+                    // W1[o][h][s][t] += conj(W2[o][kp+1][u][s]) G[(kp,u)][(k,t)]
+                    const C64 w2v = w2[(o * 9 + kp + 1) * 16 + u * 4 + s];
+                    const C64 g2v = G[(kp * 4 + u) * 32 + k * 4 + t];
+                    a1.re += w2v.re * g2v.re + w2v.im * g2v.im;
+                    a1.im += w2v.re * g2v.im - w2v.im * g2v.re;
+                }
+        }
+        g2[e].re += 2.0 * a2.re;
+        g2[e].im += 2.0 * a2.im;
+        g1[e].re += 2.0 * a1.re;
+        g1[e].im += 2.0 * a1.im;
+    }
+}
+
+// collapsed stencil: M0 = sum_o W2[o][0] W1[o][0], M[(k,s)][(k',t)] = sum_o (W2[o][k+1] W1[o][k'+1])[s][t]
+__global__ void k_collapse(Cx<float> *mcol, const C64 *w1, const C64 *w2, int C) {
+    for (int e = threadIdx.x; e < 1040; e += blockDim.x) {
+        int hA, hB, s, t;
+        if (e < 1024) {
+            const int i = e / 32, j = e % 32;
+            hA = (i >> 2) + 1;
+            s = i & 3;
+            hB = (j >> 2) + 1;
+            t = j & 3;
+        } else {
+            hA = hB = 0;
+            s = (e - 1024) >> 2;
+            t = (e - 1024) & 3;
+        }
+        C64 acc = {0.0, 0.0};
+        for (int o = 0; o < C; ++o)
+            for (int u = 0; u < 4; ++u) acc = acc + w2[(o * 9 + hA) * 16 + s * 4 + u] * w1[(o * 9 + hB) * 16 + u * 4 + t];
+        mcol[e] = {float(acc.re), float(acc.im)};
+    }
+}
+
+__global__ void k_adam(C64 *w, C64 *m, C64 *v, const C64 *g, int n, double lr, double b1, double b2, double eps,
+                       double c1, double c2, double inv_batches) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double gr = g[i].re * inv_batches, gi = g[i].im * inv_batches;
+    m[i].re = b1 * m[i].re + (1 - b1) * gr;
+    m[i].im = b1 * m[i].im + (1 - b1) * gi;
+    v[i].re = b2 * v[i].re + (1 - b2) * gr * gr;
+    v[i].im = b2 * v[i].im + (1 - b2) * gi * gi;
+    w[i].re -= lr * (m[i].re / c1) / (sqrt(v[i].re / c2) + eps);
+    w[i].im -= lr * (m[i].im / c1) / (sqrt(v[i].im / c2) + eps);
+}
+
+__global__ __launch_bounds__(256) void k_diff_scale(C32 *__restrict__ d, const C32 *__restrict__ phi,
+                                                   const C32 *__restrict__ tgt, const double *__restrict__ inv_n,
+                                                   long N, int R) {
+    const long stride = long(gridDim.x) * blockDim.x;
+    for (long e = long(blockIdx.x) * blockDim.x + threadIdx.x; e < N; e += stride) {
+        const float sc = float(inv_n[e % R]);
+        const C32 a = phi[e], b = tgt[e];
+        d[e] = {(a.re - b.re) * sc, (a.im - b.im) * sc};
+    }
+}
+
+template <typename F>
+hipError_t dispatch_nc(int Nc, F &&f) {
+    switch (Nc) {
+        case 2: return f(std::integral_constant<int, 2>{});
+        case 3: return f(std::integral_constant<int, 3>{});
+        case 4: return f(std::integral_constant<int, 4>{});
+        case 8: return f(std::integral_constant<int, 8>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+void *dmal(size_t bytes, hipError_t &err) {
+    void *p = nullptr;
+    if (err == hipSuccess) err = hipMalloc(&p, bytes);
+    return p;
+}
+
+}  // namespace
+
+LgnnPreconditioner::LgnnPreconditioner(const Geom &g, int Nc, int R, int channels)
+    : g_(g), Nc_(Nc), R_(R), C_(channels), n_(long(g.Vh) * 4 * Nc) {}
+
+LgnnPreconditioner::~LgnnPreconditioner() {
+    for (void *p : {(void *)w1_, (void *)w2_, (void *)g1_, (void *)g2_, (void *)m1_, (void *)m2_, (void *)v1_,
+                    (void *)v2_, (void *)mcol_, u_, q_, inf_, phi_, dlt_, (void *)gm_, (void *)dnorm_,
+                    (void *)dloss_, gauge32_})
+        if (p) (void)hipFree(p);
+    if (hbuf_) (void)hipHostFree(hbuf_);
+    reduce_work_free(red_);
+}
+
+bool LgnnPreconditioner::supports(int prec) const {
+    return prec == kDouble || prec == kSingle || prec == kHalf || prec == kBF16;
+}
+
+hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
+    if (u_) return hipSuccess;
+    hipError_t err = hipSuccess;
+    const size_t half32 = size_t(n_) * R_ * sizeof(C32);
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
+    u_ = dmal(8 * half32, err);
+    q_ = dmal(8 * half32, err);
+    inf_ = dmal(half32, err);
+    phi_ = dmal(half32, err);
+    dlt_ = dmal(half32, err);
+    gram_blocks_ = 512;
+    gm_ = static_cast<C64 *>(dmal(sizeof(C64) * 1040 * size_t(gram_blocks_), err));
+    mcol_ = static_cast<Cx<float> *>(dmal(sizeof(Cx<float>) * 1040, err));
+    w1_ = static_cast<C64 *>(dmal(wbytes, err));
+    w2_ = static_cast<C64 *>(dmal(wbytes, err));
+    g1_ = static_cast<C64 *>(dmal(wbytes, err));
+    g2_ = static_cast<C64 *>(dmal(wbytes, err));
+    m1_ = static_cast<C64 *>(dmal(wbytes, err));
+    m2_ = static_cast<C64 *>(dmal(wbytes, err));
+    v1_ = static_cast<C64 *>(dmal(wbytes, err));
+    v2_ = static_cast<C64 *>(dmal(wbytes, err));
+    dnorm_ = static_cast<double *>(dmal(sizeof(double) * 2 * R_, err));
+    dloss_ = static_cast<double *>(dmal(sizeof(double) * 2 * R_, err));
+    if (err != hipSuccess) {
+        set_error(5, "learned preconditioner: allocation failed (%s)", hipGetErrorString(err));
+        return err;
+    }
+    QCU_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&hbuf_), sizeof(double) * 4 * R_));
+    QCU_HIP_TRY(reduce_work_alloc(red_, R_));
+    for (C64 *p : {g1_, g2_, m1_, m2_, v1_, v2_}) QCU_HIP_TRY(hipMemsetAsync(p, 0, wbytes, s));
+    return hipSuccess;
+}
+
+hipError_t LgnnPreconditioner::set_gauge(const void *gauge64, hipStream_t s) {
+    gauge_ = gauge64;
+    const long count = 8L * g_.Vh * Nc_ * Nc_;
+    if (!gauge32_) QCU_HIP_TRY(hipMalloc(&gauge32_, sizeof(C32) * count));
+    return launch_convert(gauge32_, kSingle, gauge64, kDouble, count, nullptr, 1, s);
+}
+
+hipError_t LgnnPreconditioner::set_weights(const double *w1, const double *w2, hipStream_t s) {
+    QCU_HIP_TRY(ensure_fields(s));
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
+    QCU_HIP_TRY(hipMemcpyAsync(w1_, w1, wbytes, hipMemcpyHostToDevice, s));
+    QCU_HIP_TRY(hipMemcpyAsync(w2_, w2, wbytes, hipMemcpyHostToDevice, s));
+    for (C64 *p : {g1_, g2_, m1_, m2_, v1_, v2_}) QCU_HIP_TRY(hipMemsetAsync(p, 0, wbytes, s));
+    adam_t_ = 0;
+    have_weights_ = true;
+    QCU_HIP_TRY(collapse(s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t LgnnPreconditioner::get_weights(double *w1, double *w2, hipStream_t s) {
+    if (!have_weights_) return hipErrorInvalidValue;
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
+    QCU_HIP_TRY(hipMemcpyAsync(w1, w1_, wbytes, hipMemcpyDeviceToHost, s));
+    QCU_HIP_TRY(hipMemcpyAsync(w2, w2_, wbytes, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t LgnnPreconditioner::collapse(hipStream_t s) {
+    hipLaunchKernelGGL(k_collapse, dim3(1), dim3(256), 0, s, mcol_, w1_, w2_, C_);
+    return hipGetLastError();
+}
+
+hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int out_prec, hipStream_t s) {
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
+    const int spb = R_ >= 256 ? 1 : 256 / R_;
+    const dim3 block(R_, spb), grid(div_up(g_.Vh, spb));
+    const long fsize = n_ * R_;
+    C32 *u = static_cast<C32 *>(u_), *q = static_cast<C32 *>(q_);
+    const C32 *g32 = static_cast<const C32 *>(gauge32_);
+    hipError_t e = dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
+        constexpr int NC = decltype(ncv)::value;
+        switch (prec) {
+            case kDouble: hipLaunchKernelGGL((k_transport8<Cx<double>, NC>), grid, block, 0, s, u, static_cast<const Cx<double> *>(in), g32, tab.nbr, g_.Vh, R_, 0); break;
+            case kSingle: hipLaunchKernelGGL((k_transport8<Cx<float>, NC>), grid, block, 0, s, u, static_cast<const Cx<float> *>(in), g32, tab.nbr, g_.Vh, R_, 0); break;
+            case kHalf: hipLaunchKernelGGL((k_transport8<Half2Store, NC>), grid, block, 0, s, u, static_cast<const Half2Store *>(in), g32, tab.nbr, g_.Vh, R_, 0); break;
+            case kBF16: hipLaunchKernelGGL((k_transport8<Bf162Store, NC>), grid, block, 0, s, u, static_cast<const Bf162Store *>(in), g32, tab.nbr, g_.Vh, R_, 0); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
+    const long ncol = long(g_.Vh) * Nc_ * R_;
+    hipLaunchKernelGGL(k_mix, dim3(int(std::min<long>((ncol + 255) / 256, 8192))), dim3(256), 0, s, q, u, mcol_, ncol,
+                       Nc_, R_, fsize);
+    QCU_HIP_TRY(hipGetLastError());
+    const C32 *m0 = mcol_ + 1024;
+    return dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
+        constexpr int NC = decltype(ncv)::value;
+#define QCU_G8(CS, CO) hipLaunchKernelGGL((k_gather8<CS, CO, NC>), grid, block, 0, s, static_cast<CO *>(out), static_cast<const CS *>(in), q, g32, tab.nbr, m0, g_.Vh, R_)
+        if (out_prec == kSingle) {
+            switch (prec) {
+                case kDouble: QCU_G8(Cx<double>, Cx<float>); break;
+                case kSingle: QCU_G8(Cx<float>, Cx<float>); break;
+                case kHalf: QCU_G8(Half2Store, Cx<float>); break;
+                case kBF16: QCU_G8(Bf162Store, Cx<float>); break;
+                default: return hipErrorInvalidValue;
+            }
+        } else {
+            if (prec != out_prec) return hipErrorInvalidValue;
+            switch (prec) {
+                case kDouble: QCU_G8(Cx<double>, Cx<double>); break;
+                case kHalf: QCU_G8(Half2Store, Half2Store); break;
+                case kBF16: QCU_G8(Bf162Store, Bf162Store); break;
+                default: return hipErrorInvalidValue;
+            }
+        }
+#undef QCU_G8
+        return hipGetLastError();
+    });
+}
+
+hipError_t LgnnPreconditioner::apply(void *out, const void *in, int prec, hipStream_t s) {
+    if (!ready()) {
+        set_error(1, "learned preconditioner has no weights or gauge field");
+        return hipErrorInvalidValue;
+    }
+    QCU_HIP_TRY(ensure_fields(s));
+    return forward(in, prec, out, prec, s);
+}
+
+hipError_t LgnnPreconditioner::zero_grad(hipStream_t s) {
+    QCU_HIP_TRY(ensure_fields(s));
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
+    QCU_HIP_TRY(hipMemsetAsync(g1_, 0, wbytes, s));
+    return hipMemsetAsync(g2_, 0, wbytes, s);
+}
+
+hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, double *loss_host, hipStream_t s) {
+    if (!ready()) return hipErrorInvalidValue;
+    QCU_HIP_TRY(ensure_fields(s));
+    const long N = n_ * R_;
+    // forward in FP32 (keeps u for the gradient)
+    QCU_HIP_TRY(forward(in, kSingle, phi_, kSingle, s));
+    // per-RHS ||e||^2, then d = (phi - e) / ||e||^2 and loss = sum ||phi - e||^2 / ||e||^2
+    QCU_HIP_TRY(launch_norm(dnorm_, target, n_, R_, kSingle, false, red_, 0, s));
+    QCU_HIP_TRY(hipMemcpyAsync(hbuf_, dnorm_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
+    QCU_HIP_TRY(hipStreamSynchronize(s));
+    for (int r = 0; r < R_; ++r) hbuf_[R_ + r] = hbuf_[r] > 0 ? 1.0 / hbuf_[r] : 0.0;
+    QCU_HIP_TRY(hipMemcpyAsync(dnorm_ + R_, hbuf_ + R_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_diff_scale, dim3(2048), dim3(256), 0, s, static_cast<C32 *>(dlt_),
+                       static_cast<const C32 *>(phi_), static_cast<const C32 *>(target), dnorm_ + R_, N, R_);
+    QCU_HIP_TRY(hipGetLastError());
+    if (loss_host) {
+        // ||d||^2 * ||e||^2 = ||phi - e||^2 / ||e||^2
+        QCU_HIP_TRY(launch_norm(dloss_, dlt_, n_, R_, kSingle, false, red_, 1, s));
+        QCU_HIP_TRY(hipMemcpyAsync(hbuf_ + 2 * R_, dloss_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
+        QCU_HIP_TRY(hipStreamSynchronize(s));
+        double L = 0.0;
+        for (int r = 0; r < R_; ++r) L += hbuf_[2 * R_ + r] * hbuf_[r];
+        *loss_host = L;
+    }
+    // w_k = T_k d on the even sites (the adjoint of hop k is hop k ^ 1), into q
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
+    const int spb = R_ >= 256 ? 1 : 256 / R_;
+    const dim3 block(R_, spb), grid(div_up(g_.Vh, spb));
+    QCU_HIP_TRY(dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
+        constexpr int NC = decltype(ncv)::value;
+        hipLaunchKernelGGL((k_transport8<Cx<float>, NC>), grid, block, 0, s, static_cast<C32 *>(q_),
+                           static_cast<const C32 *>(dlt_), static_cast<const C32 *>(gauge32_), tab.nbr, g_.Vh, R_, 0);
+        return hipGetLastError();
+    }));
+    const long ncol = long(g_.Vh) * Nc_ * R_;
+    const long fsize = n_ * R_;
+    hipLaunchKernelGGL(k_gram, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(q_),
+                       static_cast<const C32 *>(u_), ncol, Nc_, R_, fsize);
+    QCU_HIP_TRY(hipGetLastError());
+    // the input itself is needed in FP32 for the M0 gradient
+    const C32 *inf = static_cast<const C32 *>(in);
+    hipLaunchKernelGGL(k_gram0, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(dlt_), inf, ncol,
+                       Nc_, R_);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_chain, dim3(1), dim3(256), 0, s, g1_, g2_, w1_, w2_, gm_, gram_blocks_, C_);
+    return hipGetLastError();
+}
+
+hipError_t LgnnPreconditioner::read_grad(double *gw1, double *gw2, hipStream_t s) {
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
+    QCU_HIP_TRY(hipMemcpyAsync(gw1, g1_, wbytes, hipMemcpyDeviceToHost, s));
+    QCU_HIP_TRY(hipMemcpyAsync(gw2, g2_, wbytes, hipMemcpyDeviceToHost, s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t LgnnPreconditioner::adam_step(double lr, int batches, hipStream_t s) {
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-12;
+    ++adam_t_;
+    const double c1 = 1.0 - std::pow(b1, adam_t_), c2 = 1.0 - std::pow(b2, adam_t_);
+    const int n = C_ * 9 * 16;
+    const double inv = batches > 0 ? 1.0 / batches : 1.0;
+    hipLaunchKernelGGL(k_adam, dim3(div_up(n, 256)), dim3(256), 0, s, w1_, m1_, v1_, g1_, n, lr, b1, b2, eps, c1, c2, inv);
+    hipLaunchKernelGGL(k_adam, dim3(div_up(n, 256)), dim3(256), 0, s, w2_, m2_, v2_, g2_, n, lr, b1, b2, eps, c1, c2, inv);
+    QCU_HIP_TRY(hipGetLastError());
+    return collapse(s);
+}
+
+}  // namespace qcu

@@ -26,6 +26,7 @@
 #include "qcu.h"
 #include "qcu_ops.h"
 #include "solver.h"
+#include "tune.h"
 
 namespace qcu {
 
@@ -60,12 +61,29 @@ struct Context {
     std::unique_ptr<BiCGStab> solver;
     SolverOptions opts;
     std::unique_ptr<LgnnPreconditioner> precond;
+    struct LgnnBatch {
+        int cfg;
+        void *e, *r;  // FP32 odd half vectors
+    };
+    std::vector<void *> lgnn_gauges;  // FP64 copies, one per collected configuration
+    std::vector<LgnnBatch> lgnn_batches;
     hipStream_t stream = nullptr;
+
+    void clear_lgnn_data() {
+        for (auto &b : lgnn_batches) {
+            (void)hipFree(b.e);
+            (void)hipFree(b.r);
+        }
+        for (void *p : lgnn_gauges) (void)hipFree(p);
+        lgnn_batches.clear();
+        lgnn_gauges.clear();
+    }
 
     long half_n() const { return long(g.Vh) * 4 * Nc; }
     long gauge_count() const { return 4L * 2 * g.Vh * Nc * Nc; }
 
     ~Context() {
+        clear_lgnn_data();
         solver.reset();
         precond.reset();
         for (void *p : {gauge64, bufA, bufB}) if (p) (void)hipFree(p);
@@ -285,6 +303,11 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
             return g_err;
         }
         pc = c->precond.get();
+        if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "preconditioner gauge")) {
+            c->q_in.clear();
+            c->q_out.clear();
+            return g_err;
+        }
     }
     bool ok = hip_ok(launch_gather(c->bufA, kDouble, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n,
                                    c->R, c->stream), "gather");
@@ -493,6 +516,229 @@ int qcuDslashTune(int variant, int by, int bz, int spw) {
     t.bz = bz;
     t.spw = spw;
     dslash_set_override(t);
+    return 0;
+}
+
+int qcuDslashAutotune(const int lattice[4], int n_color, int m_rhs, int precision, int repeats, int best[4],
+                      double *best_us, double *records, int max_records, int *n_records) {
+    if (!lattice || lattice[0] % 2 || !dslash_supported(n_color, m_rhs) || !valid_storage(precision) || repeats < 1) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashAutotune: invalid arguments");
+        return g_err;
+    }
+    DslashTune t;
+    double us = 0.0;
+    std::vector<TuneRecord> log;
+    if (!hip_ok(dslash_autotune(lattice, n_color, m_rhs, precision, repeats, t, us, &log), "autotune")) return g_err;
+    if (best) {
+        best[0] = t.variant;
+        best[1] = t.by;
+        best[2] = t.bz;
+        best[3] = t.spw;
+    }
+    if (best_us) *best_us = us;
+    int n = 0;
+    for (const auto &r : log) {
+        if (records && n < max_records) {
+            double *o = records + 5 * n;
+            o[0] = r.tune.variant;
+            o[1] = r.tune.by;
+            o[2] = r.tune.bz;
+            o[3] = r.tune.spw;
+            o[4] = r.us;
+        }
+        ++n;
+    }
+    if (n_records) *n_records = n;
+    return 0;
+}
+
+int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
+                      int spw) {
+    if (!lattice || variant < kDslashLaneRhs || variant > kDslashMfma || by < 0 || bz < 0 || spw < 0) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashSetTuned: invalid arguments");
+        return g_err;
+    }
+    DslashTune t;
+    t.variant = variant;
+    t.by = by;
+    t.bz = bz;
+    t.spw = spw;
+    dslash_set_tuned(lattice, n_color, m_rhs, precision, t);
+    return 0;
+}
+
+// ---- learned preconditioner --------------------------------------------
+int qcuLgnnSetWeights(int channels, const double *w1, const double *w2) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (channels < 1 || channels > 64 || !w1 || !w2) {
+        set_error(kQcuErrorInvalidValue, "qcuLgnnSetWeights: invalid arguments");
+        return g_err;
+    }
+    if (!c->precond || c->precond->channels() != channels)
+        c->precond = std::make_unique<LgnnPreconditioner>(c->g, c->Nc, c->R, channels);
+    if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge")) return g_err;
+    return hip_ok(c->precond->set_weights(w1, w2, c->stream), "lgnn weights") ? 0 : g_err;
+}
+
+int qcuLgnnGetWeights(double *w1, double *w2) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->precond) {
+        set_error(kQcuErrorNotInitialized, "no learned preconditioner");
+        return g_err;
+    }
+    return hip_ok(c->precond->get_weights(w1, w2, c->stream), "lgnn weights") ? 0 : g_err;
+}
+
+int qcuLgnnApply(void *out, const void *in, int precision) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->precond || !valid_storage(precision)) {
+        set_error(kQcuErrorNotInitialized, "qcuLgnnApply: no preconditioner or invalid precision");
+        return g_err;
+    }
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    ok = ok && hip_ok(c->precond->apply(out, in, precision, c->stream), "lgnn apply");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn apply");
+    return ok ? 0 : g_err;
+}
+
+int qcuLgnnLossGrad(const void *in, const void *target, double *loss, double *gw1, double *gw2) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->precond) {
+        set_error(kQcuErrorNotInitialized, "no learned preconditioner");
+        return g_err;
+    }
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn grad");
+    ok = ok && hip_ok(c->precond->loss_grad(in, target, loss, c->stream), "lgnn grad");
+    if (ok && gw1 && gw2) ok = hip_ok(c->precond->read_grad(gw1, gw2, c->stream), "lgnn grad");
+    return ok ? 0 : g_err;
+}
+
+int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batches) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->have_dslash) {
+        set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
+        return g_err;
+    }
+    if (!queue_full(c) || n_snapshots < 1) return g_err ? g_err : kQcuErrorInvalidValue;
+    const long n = c->half_n();
+    const long nR = n * c->R;
+    const size_t b64 = size_t(nR) * sizeof(Cx<double>), b32 = size_t(nR) * sizeof(Cx<float>);
+    SolverOptions opt;
+    opt.inner_prec = kDouble;
+    GaugeSet gs;
+    gs.g64 = c->gauge64;
+    SolveResult res;
+    bool ok = hip_ok(launch_gather(c->bufA, kDouble, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n, c->R,
+                                   c->stream), "gather");
+    c->q_in.clear();
+    c->q_out.clear();
+    // pass 1: iteration count
+    ok = ok && hip_ok(c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, opt, nullptr, res,
+                                       c->stream), "collect solve");
+    if (!ok) return g_err;
+    const int iters = res.iterations;
+    std::vector<int> pick;
+    for (int k = 0; k < n_snapshots; ++k) {
+        const int it = 1 + int((long(k) * std::max(iters - 1, 1)) / n_snapshots);
+        if (pick.empty() || pick.back() != it) pick.push_back(it);
+    }
+    // pass 2: the same (deterministic) iterates, FP64 snapshots at the picked iterations
+    std::vector<void *> snaps(pick.size(), nullptr);
+    for (auto &p : snaps)
+        if (!hip_ok(hipMalloc(&p, b64), "snapshot")) ok = false;
+    size_t next = 0;
+    c->solver->iterate_hook = [&](int it, const void *x, hipStream_t s) -> hipError_t {
+        if (next < pick.size() && it == pick[next]) return hipMemcpyAsync(snaps[next++], x, b64, hipMemcpyDeviceToDevice, s);
+        return hipSuccess;
+    };
+    ok = ok && hip_ok(c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, opt, nullptr, res,
+                                       c->stream), "collect solve");
+    c->solver->iterate_hook = nullptr;
+    // gauge copy for this configuration
+    void *gcopy = nullptr;
+    ok = ok && hip_ok(hipMalloc(&gcopy, size_t(c->gauge_count()) * sizeof(Cx<double>)), "gauge copy");
+    ok = ok && hip_ok(hipMemcpyAsync(gcopy, c->gauge64, size_t(c->gauge_count()) * sizeof(Cx<double>),
+                                     hipMemcpyDeviceToDevice, c->stream), "gauge copy");
+    const int cfg = int(c->lgnn_gauges.size());
+    if (ok) c->lgnn_gauges.push_back(gcopy);
+    const Cx<double> *xstar = static_cast<const Cx<double> *>(c->bufB) + nR;  // odd half of x
+    void *d64 = nullptr, *r64 = nullptr, *tmp = nullptr;
+    ok = ok && hip_ok(hipMalloc(&d64, b64), "alloc") && hip_ok(hipMalloc(&r64, b64), "alloc") &&
+         hip_ok(hipMalloc(&tmp, b64), "alloc");
+    C64 *minus1 = nullptr;
+    ok = ok && hip_ok(hipMalloc(&minus1, sizeof(C64) * c->R), "alloc");
+    ok = ok && hip_ok(launch_scalar_fill(minus1, C64{-1.0, 0.0}, c->R, c->stream), "fill");
+    for (size_t k = 0; ok && k < next; ++k) {
+        // e = x* - x_k ;  r = A e  (odd Schur operator, FP64)
+        AxpyArgs ax{kXpay, d64, xstar, snaps[k], nullptr, minus1, nullptr, nullptr, n, c->R, kDouble};
+        ok = hip_ok(launch_axpy(ax, c->stream), "pair");
+        ok = ok && hip_ok(c->solver->apply_schur(r64, d64, tmp, c->gauge64, kDouble, c->kappa, 0, c->stream), "pair");
+        Context::LgnnBatch bt{cfg, nullptr, nullptr};
+        ok = ok && hip_ok(hipMalloc(&bt.e, b32), "pair") && hip_ok(hipMalloc(&bt.r, b32), "pair");
+        ok = ok && hip_ok(launch_convert(bt.e, kSingle, d64, kDouble, nR, nullptr, c->R, c->stream), "pair");
+        ok = ok && hip_ok(launch_convert(bt.r, kSingle, r64, kDouble, nR, nullptr, c->R, c->stream), "pair");
+        if (ok) c->lgnn_batches.push_back(bt);
+    }
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "collect");
+    for (void *p : snaps)
+        if (p) (void)hipFree(p);
+    for (void *p : {d64, r64, tmp, (void *)minus1})
+        if (p) (void)hipFree(p);
+    if (n_batches) *n_batches = int(c->lgnn_batches.size());
+    return ok ? 0 : g_err;
+}
+
+int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_per_epoch) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->precond || c->lgnn_batches.empty() || epochs < 1) {
+        set_error(kQcuErrorNotInitialized, "qcuLgnnTrain: needs weights (qcuLgnnSetWeights) and collected data");
+        return g_err;
+    }
+    std::vector<size_t> order(c->lgnn_batches.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    bool ok = true;
+    for (int ep = 0; ok && ep < epochs; ++ep) {
+        for (size_t i = order.size(); i > 1; --i) {  // Fisher-Yates, splitmix64
+            st += 0x9E3779B97F4A7C15ull;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            std::swap(order[i - 1], order[size_t(z % i)]);
+        }
+        double total = 0.0;
+        int cur = -1;
+        for (size_t i = 0; ok && i < order.size(); ++i) {
+            const auto &bt = c->lgnn_batches[order[i]];
+            if (bt.cfg != cur) {
+                ok = hip_ok(c->precond->set_gauge(c->lgnn_gauges[bt.cfg], c->stream), "lgnn gauge");
+                cur = bt.cfg;
+            }
+            double L = 0.0;
+            ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn train");
+            ok = ok && hip_ok(c->precond->loss_grad(bt.r, bt.e, &L, c->stream), "lgnn train");
+            ok = ok && hip_ok(c->precond->adam_step(learning_rate, 1, c->stream), "lgnn train");
+            total += L;
+        }
+        if (loss_per_epoch) loss_per_epoch[ep] = total / (double(order.size()) * c->R);
+    }
+    ok = ok && hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn train");
+    return ok ? 0 : g_err;
+}
+
+int qcuLgnnClearData(void) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    c->clear_lgnn_data();
     return 0;
 }
 

@@ -1,6 +1,7 @@
 // solver.h -- even/odd preconditioned multi-RHS BiCGStab on gfx950.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "blas.h"
@@ -61,6 +62,10 @@ public:
     // out = in - kappa^2 D_oe D_eo in in precision `prec` (tmp: half vector)
     hipError_t apply_schur(void *out, const void *in, void *tmp, const void *gauge, int prec,
                            double kappa, int dagger, hipStream_t s);
+
+    // Called after every iteration of a plain (FP64) solve with the iteration
+    // number and the current FP64 odd iterate x_o (device, [Vh][4][Nc][R]).
+    std::function<hipError_t(int, const void *, hipStream_t)> iterate_hook;
 
     const Geom &geom() const { return g_; }
     int nc() const { return Nc_; }

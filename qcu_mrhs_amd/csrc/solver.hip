@@ -210,7 +210,8 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
     QCU_HIP_TRY(ensure(P));
     if (prec && !ph_) {
         hipError_t err = hipSuccess;
-        const size_t half = size_t(n_) * R_ * 2 * prec_bytes(P);
+        // sized for FP64 so a later solve in a wider inner precision fits
+        const size_t half = size_t(n_) * R_ * sizeof(Cx<double>);
         ph_ = dmalloc(half, err);
         sh_ = dmalloc(half, err);
         if (err != hipSuccess) return err;
@@ -278,6 +279,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
         QCU_HIP_TRY(hipMemcpyAsync(h_, st_ + kRnorm2 * R_, sizeof(C64) * R_, hipMemcpyDeviceToHost, s));
         QCU_HIP_TRY(hipStreamSynchronize(s));
         res.iterations = it + 1;
+        if (iterate_hook && !mixed) QCU_HIP_TRY(iterate_hook(it + 1, x_o, s));
         bool conv_est = true, bad = false, want_update = false;
         for (int r = 0; r < R_; ++r) {
             const double r2 = h_[2 * r];
