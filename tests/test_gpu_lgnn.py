@@ -1,0 +1,186 @@
+"""GPU parity of the learned preconditioner (qcu_mrhs_amd/csrc/lgnn.hip)
+against the CPU oracle (oracle/lgnn.py), through the C entry points, and what
+the trained network gains over the solver without it.
+
+The GPU evaluates the network with FP32 transports and an FP32 collapsed
+stencil (FP64 partial sums for the gradient reductions), so forward results
+agree with the FP64 oracle to a relative 1e-5 and gradients to 1e-4."""
+import numpy as np
+import pytest
+
+from oracle import lgnn as olgnn
+from oracle import oracle
+from qcu_mrhs_amd import api, fields, lgnn
+
+pytestmark = pytest.mark.gpu
+
+L, Nc, R, KAPPA = (4, 4, 4, 8), 3, 4, 0.13
+RTOL_FWD = 1e-5
+RTOL_GRAD = 1e-4
+
+
+def _case(*a):
+    print("case", *a, flush=True)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _setup(L_, Nc_, R_, gauge, kappa, inner=2):
+    api.initGridSize(L_, Nc_, R_, 2, inner)
+    d_g = api.DeviceArray.from_complex(gauge.reshape(-1))
+    api.loadQcuGauge(d_g, 2)
+    api.getDslash(0, 1.0 / (2 * kappa) - 4.0)
+    return d_g
+
+
+def _odd_field(seed, shape):
+    g = np.random.default_rng(seed)
+    return g.standard_normal(shape) + 1j * g.standard_normal(shape)
+
+
+def _abi_weights(net):
+    """oracle W1 [C][1][9][4][4], W2 [1][C][9][4][4] -> C ABI [C][9][4][4] each."""
+    return net.W1[:, 0].copy(), net.W2[0].copy()
+
+
+@pytest.mark.parametrize("prec,tol", [(2, RTOL_FWD), (1, RTOL_FWD), (0, 2e-3), (4, 1.5e-2)])
+def test_forward_matches_oracle(prec, tol):
+    _case("lgnn forward", prec)
+    gauge = fields.gauge_field(L, Nc, 71, kind="haar")
+    d_g = _setup(L, Nc, R, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    psi = _odd_field(72, (Vh, 4, Nc, R))
+    net = olgnn.Net.random(8, 73, scale=0.3)
+    lgnn.set_weights(*_abi_weights(net))
+    d_in = api.DeviceArray.from_complex(psi.reshape(-1), prec)
+    d_out = api.DeviceArray(api.storage_bytes(psi.size, prec))
+    lgnn.apply(d_out, d_in, prec)
+    got = d_out.complex(psi.size, prec)
+    # the oracle sees the input as stored (rounded to the storage precision)
+    psi_s = api.from_storage(api.to_storage(psi.reshape(-1), prec), prec).reshape(psi.shape)
+    ref, _ = net.forward(olgnn.Transport(gauge, L, Nc), psi_s, L, Nc, R)
+    err = _rel(got, ref.reshape(-1))
+    print(f"precision {prec}: forward relative error {err:.2e}")
+    assert err < tol
+    api.finalizeQcu()
+    del d_g
+
+
+def test_neumann_weights_reproduce_schur_neumann_term():
+    """Weights (1 -+ gamma_mu) built by the package (not the oracle): the GPU
+    network equals psi + kappa^2 D_oe D_eo psi = 2 psi - A psi, with A the odd
+    Schur operator of the FP64 dslash oracle."""
+    _case("lgnn neumann")
+    gauge = fields.gauge_field(L, Nc, 81, kind="haar")
+    d_g = _setup(L, Nc, R, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    psi = _odd_field(82, (Vh, 4, Nc, R)).reshape(-1)
+    lgnn.set_weights(*lgnn.neumann_weights(KAPPA, channels=8))
+    d_in = api.DeviceArray.from_complex(psi)
+    d_out = api.DeviceArray(psi.nbytes)
+    lgnn.apply(d_out, d_in, 2)
+    got = d_out.complex(psi.size)
+    ref = 2 * psi - oracle.schur(psi, gauge, L, Nc, R, KAPPA)
+    assert _rel(got, ref) < RTOL_FWD
+    api.finalizeQcu()
+    del d_g
+
+
+def test_loss_and_gradients_match_oracle():
+    _case("lgnn gradient")
+    gauge = fields.gauge_field(L, Nc, 91, kind="haar")
+    d_g = _setup(L, Nc, R, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    shape = (Vh, 4, Nc, R)
+    r = _odd_field(92, shape).astype(np.complex64).astype(np.complex128)
+    e = _odd_field(93, shape).astype(np.complex64).astype(np.complex128)
+    net = olgnn.Net.random(8, 94, scale=0.3)
+    lgnn.set_weights(*_abi_weights(net))
+    d_r = api.DeviceArray.from_complex(r.reshape(-1), 1)
+    d_e = api.DeviceArray.from_complex(e.reshape(-1), 1)
+    loss, g1, g2 = lgnn.loss_grad(d_r, d_e, 8)
+    ref_loss, ref_g1, ref_g2 = net.loss_and_grad(olgnn.Transport(gauge, L, Nc), r, e, L, Nc, R)
+    print(f"loss {loss:.8e} (oracle {ref_loss:.8e}); gradient errors "
+          f"{_rel(g1, ref_g1[:, 0]):.2e}, {_rel(g2, ref_g2[0]):.2e}")
+    assert abs(loss - ref_loss) <= RTOL_FWD * abs(ref_loss)
+    assert _rel(g1, ref_g1[:, 0]) < RTOL_GRAD
+    assert _rel(g2, ref_g2[0]) < RTOL_GRAD
+    # a second call on the same batch gives the same numbers (gradients are not accumulated)
+    loss2, g1b, _ = lgnn.loss_grad(d_r, d_e, 8)
+    assert loss2 == loss and np.array_equal(g1b, g1)
+    api.finalizeQcu()
+    del d_g
+
+
+def _solve(Lt, gauge, b, kappa, use_pc, inner=2, tol=1e-10):
+    R_ = b.shape[0]
+    api.set_solver_options(inner, 0.1, use_pc)
+    bs = [api.DeviceArray.from_complex(b[k]) for k in range(R_)]
+    xs = [api.DeviceArray(b[k].nbytes) for k in range(R_)]
+    for k in range(R_):
+        api.pushBackFermions(xs[k], bs[k])
+    info = api.qcuInvertEx(5000, tol)
+    x = np.stack([xs[k].complex(b.shape[1]) for k in range(R_)])
+    return x, info
+
+
+def test_training_lowers_loss_and_iterations_on_held_out_config():
+    """8^4 SU(3), exp(i 0.3 H) links, kappa 0.135, R = 4: pairs from plain
+    FP64 solves on 4 configurations, Adam on the GPU, then the trained network
+    as a right preconditioner on a held-out configuration.  Checked: the
+    training loss falls, the preconditioned solve reaches a FP64 true residual
+    of 1e-10 (through the oracle operator) and takes fewer iterations than
+    the solve without the network and than the untrained initialisation."""
+    Lt, kappa, Rt, C = (8, 8, 8, 8), 0.135, 4, 8
+    _case("lgnn train", Lt)
+    gauges = [fields.gauge_field(Lt, Nc, 500 + k, kind="smooth", eps=0.3) for k in range(5)]
+    d_g = _setup(Lt, Nc, Rt, gauges[0], kappa)
+    w1, w2 = lgnn.neumann_weights(kappa, channels=C, noise=1e-3, seed=5)
+    lgnn.set_weights(w1, w2)
+    lgnn.clear_data()
+    nb = 0
+    for k in range(4):
+        d_g.upload(gauges[k].reshape(-1))
+        api.loadQcuGauge(d_g, 2)
+        b = fields.gaussian_sources(Lt, Nc, Rt, 600 + k)
+        bs = [api.DeviceArray.from_complex(b[j]) for j in range(Rt)]
+        xs = [api.DeviceArray(b[j].nbytes) for j in range(Rt)]
+        for j in range(Rt):
+            api.pushBackFermions(xs[j], bs[j])
+        nb = lgnn.collect(5000, 1e-10, 16)
+    assert nb == 64, nb
+    losses = lgnn.train(40, 2e-3, seed=3)
+    print("training loss per epoch:", " ".join(f"{v:.4f}" for v in losses))
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < 0.9 * losses[0]
+    trained = lgnn.get_weights(C)
+    lgnn.clear_data()
+
+    # held-out configuration
+    d_g.upload(gauges[4].reshape(-1))
+    api.loadQcuGauge(d_g, 2)
+    b = fields.gaussian_sources(Lt, Nc, Rt, 700)
+    _, plain = _solve(Lt, gauges[4], b, kappa, False)
+    lgnn.set_weights(w1, w2)
+    _, init = _solve(Lt, gauges[4], b, kappa, True)
+    lgnn.set_weights(*trained)
+    x, pc = _solve(Lt, gauges[4], b, kappa, True)
+    bm = fields.to_mrhs(b, Nc)
+    r = bm - oracle.mat(fields.to_mrhs(x, Nc), gauges[4], Lt, Nc, Rt, kappa)
+    true_rel = (oracle.norm(r, Rt) / oracle.norm(bm, Rt)).max()
+    print(f"held-out iterations: plain {plain['iterations']}, Neumann init {init['iterations']}, "
+          f"trained {pc['iterations']}; trained true residual {true_rel:.2e}")
+    assert plain["converged"] and init["converged"] and pc["converged"]
+    assert true_rel < 1.5e-10 and pc["true_rel_residual"] < 1.5e-10
+    assert pc["iterations"] < plain["iterations"]
+    assert pc["iterations"] <= init["iterations"]
+    # mixed precision (FP16 inner) with the trained network converges to the same bar
+    x16, pc16 = _solve(Lt, gauges[4], b, kappa, True, inner=0)
+    r = bm - oracle.mat(fields.to_mrhs(x16, Nc), gauges[4], Lt, Nc, Rt, kappa)
+    true16 = (oracle.norm(r, Rt) / oracle.norm(bm, Rt)).max()
+    print(f"FP16 inner + trained network: {pc16['iterations']} iterations, "
+          f"{pc16['reliable_updates']} reliable updates, true residual {true16:.2e}")
+    assert pc16["converged"] and true16 < 1.5e-10
+    api.finalizeQcu()
