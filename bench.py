@@ -82,6 +82,22 @@ def time_dslash(L, Nc, R, d_src_even, d_gauge64, prec, reps=20):
     return ms
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dslash_24c3x48_r16_hbm_traffic.md")
+
+
+def measured_traffic(workload, prec_name, variant):
+    """HBM bytes per dslash launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-kernel calibrated), for the workload and the
+    automatic kernel choice it was measured on; None otherwise."""
+    if workload != "c3" or variant != "auto" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == prec_name:
+            return int(f[2])
+    return None
+
+
 def solve_once(d_b, d_x, R, max_iter, tol):
     for r in range(R):
         api.pushBackFermions(d_x[r], d_b[r])
@@ -231,10 +247,14 @@ def main():
     dom = {2: "double", 1: "single", 0: "half", 4: "bf16"}[head_prec]
     if dom not in dslash:
         dom = next(iter(dslash))
+    kname = {"double": "k_dslash_wave<double>", "single": "k_dslash_wave<float>",
+             "half": "k_dslash_mfma<__half>", "bf16": "k_dslash_mfma<__hip_bfloat16>"}[dom]
     roof = {"bound": "hbm", "achieved": dslash[dom]["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": dslash[dom]["frac_of_8TBps"], "traffic": None,
-            "kernel": f"k_wilson_dslash ({dom} storage)", "algorithmic_bytes": dslash[dom]["bytes"],
-            "avg_ms": dslash[dom]["ms"]}
+            "frac": dslash[dom]["frac_of_8TBps"],
+            "traffic": measured_traffic(args.workload, dom, args.dslash_variant),
+            "kernel": f"{kname} one-parity dslash ({dom} storage)" if args.dslash_variant == "auto"
+                      else f"{args.dslash_variant} dslash ({dom} storage)",
+            "algorithmic_bytes": dslash[dom]["bytes"], "avg_ms": dslash[dom]["ms"]}
 
     cpu = None
     if not args.no_cpu and plain is not None:

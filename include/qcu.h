@@ -115,6 +115,9 @@ typedef struct QcuSolveInfo_s {
     int restarts;              /* breakdown restarts                          */
     double true_rel_residual;  /* max_r ||b_r - M x_r|| / ||b_r||, FP64, full */
     double seconds;            /* wall time of the solve (gather..scatter)    */
+    int preconditioned;        /* 1 if the learned preconditioner produced x  */
+    int fallback;              /* 1 if the preconditioned solve failed and
+                                  plain BiCGStab (from x0 = 0) produced x     */
 } QcuSolveInfo;
 
 /* qcuInvert with status: returns kQcuSuccess or an error code, fills info. */
@@ -123,7 +126,9 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info);
 /* Solver options.  inner_precision: precision of the inner (sloppy) solve,
  * overriding initGridSize's dslashFloatPrecision; delta: reliable-update
  * threshold (update when ||r|| < delta * max ||r|| since the last update);
- * use_preconditioner: 1 = apply the loaded learned right preconditioner. */
+ * use_preconditioner: 1 = apply the loaded learned right preconditioner; if
+ * that solve breaks down, diverges or does not converge, plain BiCGStab
+ * solves the same system again and QcuSolveInfo.fallback is set. */
 int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditioner);
 
 /* ---- learned preconditioner (no reference counterpart) ---------------
@@ -144,12 +149,14 @@ int qcuLgnnApply(void *out, const void *in, int precision);
 int qcuLgnnLossGrad(const void *in, const void *target, double *loss, double *gw1, double *gw2);
 /* Solve the queued systems with plain FP64 BiCGStab and keep n_snapshots
  * (error, residual) pairs per RHS -- e_k = x - x_k, r_k = A e_k of the odd
- * Schur system at evenly spaced iterations -- as training data for the
- * current gauge field.  *n_batches receives the total number of stored
- * batches (R pairs each).  The queue is consumed. */
-int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batches);
+ * Schur system at evenly spaced iterations -- plus n_random pairs with e a
+ * complex Gaussian vector (drawn from `seed`) and r = A e, as training data
+ * for the current gauge field.  *n_batches receives the total number of
+ * stored batches (R pairs each).  The queue is consumed. */
+int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int n_random, unsigned seed, int *n_batches);
 /* Adam over the stored batches (one step per batch, order shuffled with
- * `seed`); loss_per_epoch[e] = mean loss per pair. */
+ * `seed`, learning rate lr_e = learning_rate (1 + cos(pi e / epochs)) / 2 in
+ * epoch e); loss_per_epoch[e] = mean loss per pair. */
 int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_per_epoch);
 int qcuLgnnClearData(void);
 

@@ -33,6 +33,8 @@ class QcuSolveInfo(ctypes.Structure):
         ("restarts", ctypes.c_int),
         ("true_rel_residual", ctypes.c_double),
         ("seconds", ctypes.c_double),
+        ("preconditioned", ctypes.c_int),
+        ("fallback", ctypes.c_int),
     ]
 
 
@@ -91,7 +93,7 @@ def load() -> ctypes.CDLL:
         "qcuLgnnApply": ([vp, vp, i], i),
         "qcuLgnnLossGrad": ([vp, vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                              ctypes.POINTER(ctypes.c_double)], i),
-        "qcuLgnnCollect": ([i, d, i, ip4], i),
+        "qcuLgnnCollect": ([i, d, i, i, ctypes.c_uint, ip4], i),
         "qcuLgnnTrain": ([i, d, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnClearData": ([], i),
         "qcuTimerStart": ([], i),

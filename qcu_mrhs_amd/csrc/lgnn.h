@@ -24,6 +24,10 @@ namespace qcu {
 
 constexpr int kLgnnHops = 9;
 
+// out[i] = complex Gaussian (unit variance per real component), a pure
+// function of (key, i): SplitMix64 counter hashing + Box-Muller.
+hipError_t launch_gaussian(Cx<double> *out, long count, uint64_t key, hipStream_t s);
+
 class LgnnPreconditioner : public Preconditioner {
 public:
     LgnnPreconditioner(const Geom &g, int Nc, int R, int channels);

@@ -337,6 +337,26 @@ __global__ __launch_bounds__(256) void k_diff_scale(C32 *__restrict__ d, const C
     }
 }
 
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_gaussian(C64 *__restrict__ out, long count, uint64_t key) {
+    const long stride = long(gridDim.x) * blockDim.x;
+    for (long i = long(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t z = splitmix64(key ^ (uint64_t(i) * 0xD1B54A32D192ED03ull));
+        const double u1 = (double((z >> 11) & ((1ull << 26) - 1)) + 1.0) * (1.0 / 67108864.0);  // (0, 1]
+        const double u2 = double(z >> 38) * (1.0 / 67108864.0);                                  // [0, 1)
+        const double rad = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincospi(2.0 * u2, &sn, &cs);
+        out[i] = {rad * cs, rad * sn};
+    }
+}
+
 template <typename F>
 hipError_t dispatch_nc(int Nc, F &&f) {
     switch (Nc) {
@@ -355,6 +375,12 @@ void *dmal(size_t bytes, hipError_t &err) {
 }
 
 }  // namespace
+
+hipError_t launch_gaussian(Cx<double> *out, long count, uint64_t key, hipStream_t s) {
+    const long nb = std::min<long>((count + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_gaussian, dim3(int(nb > 0 ? nb : 1)), dim3(256), 0, s, out, count, key);
+    return hipGetLastError();
+}
 
 LgnnPreconditioner::LgnnPreconditioner(const Geom &g, int Nc, int R, int channels)
     : g_(g), Nc_(Nc), R_(R), C_(channels), n_(long(g.Vh) * 4 * Nc) {}

@@ -312,9 +312,21 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
     bool ok = hip_ok(launch_gather(c->bufA, kDouble, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n,
                                    c->R, c->stream), "gather");
     int code = kQcuSuccess;
+    int fallback = 0;
     if (ok) {
-        const hipError_t e = c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, c->opts, pc,
-                                              res, c->stream);
+        hipError_t e = c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, c->opts, pc, res,
+                                        c->stream);
+        if (pc && (e == hipErrorUnknown || (e == hipSuccess && !res.converged))) {
+            // the learned preconditioner failed (breakdown, divergence or no convergence): plain
+            // BiCGStab from x0 = 0 on the same system decides the result
+            if (g_err == kQcuErrorBreakdown) qcuClearError();
+            const int used = res.iterations;
+            e = c->solver->solve(c->bufA, c->bufB, gs, c->kappa, max_iteration, tol, c->opts, nullptr, res,
+                                 c->stream);
+            res.iterations += used;
+            res.restarts += 1;
+            fallback = 1;
+        }
         if (e != hipSuccess) {
             ok = false;
             code = g_err ? g_err : kQcuErrorDevice;
@@ -339,6 +351,8 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
         info->restarts = res.restarts;
         info->true_rel_residual = res.true_rel_residual;
         info->seconds = secs;
+        info->preconditioned = pc && !fallback ? 1 : 0;
+        info->fallback = fallback;
     }
     return code;
 }
@@ -618,14 +632,18 @@ int qcuLgnnLossGrad(const void *in, const void *target, double *loss, double *gw
     return ok ? 0 : g_err;
 }
 
-int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batches) {
+int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int n_random, unsigned seed, int *n_batches) {
     Context *c = ctx_or_error();
     if (!c) return g_err;
     if (!c->have_dslash) {
         set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
         return g_err;
     }
-    if (!queue_full(c) || n_snapshots < 1) return g_err ? g_err : kQcuErrorInvalidValue;
+    if (n_snapshots < 0 || n_random < 0 || n_snapshots + n_random < 1) {
+        set_error(kQcuErrorInvalidValue, "qcuLgnnCollect: need at least one snapshot or random pair");
+        return g_err;
+    }
+    if (!queue_full(c)) return g_err;
     const long n = c->half_n();
     const long nR = n * c->R;
     const size_t b64 = size_t(nR) * sizeof(Cx<double>), b32 = size_t(nR) * sizeof(Cx<float>);
@@ -674,10 +692,16 @@ int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batche
     C64 *minus1 = nullptr;
     ok = ok && hip_ok(hipMalloc(&minus1, sizeof(C64) * c->R), "alloc");
     ok = ok && hip_ok(launch_scalar_fill(minus1, C64{-1.0, 0.0}, c->R, c->stream), "fill");
-    for (size_t k = 0; ok && k < next; ++k) {
-        // e = x* - x_k ;  r = A e  (odd Schur operator, FP64)
-        AxpyArgs ax{kXpay, d64, xstar, snaps[k], nullptr, minus1, nullptr, nullptr, n, c->R, kDouble};
-        ok = hip_ok(launch_axpy(ax, c->stream), "pair");
+    for (size_t k = 0; ok && k < next + size_t(n_random); ++k) {
+        if (k < next) {
+            // e = x* - x_k ;  r = A e  (odd Schur operator, FP64)
+            AxpyArgs ax{kXpay, d64, xstar, snaps[k], nullptr, minus1, nullptr, nullptr, n, c->R, kDouble};
+            ok = hip_ok(launch_axpy(ax, c->stream), "pair");
+        } else {
+            // e = complex Gaussian vector, r = A e: pins the network on the high modes too
+            const uint64_t key = (uint64_t(seed) << 32) ^ (uint64_t(cfg) << 20) ^ uint64_t(k);
+            ok = hip_ok(launch_gaussian(static_cast<Cx<double> *>(d64), nR, key, c->stream), "pair");
+        }
         ok = ok && hip_ok(c->solver->apply_schur(r64, d64, tmp, c->gauge64, kDouble, c->kappa, 0, c->stream), "pair");
         Context::LgnnBatch bt{cfg, nullptr, nullptr};
         ok = ok && hip_ok(hipMalloc(&bt.e, b32), "pair") && hip_ok(hipMalloc(&bt.r, b32), "pair");
@@ -688,7 +712,7 @@ int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int *n_batche
     ok = ok && hip_ok(hipStreamSynchronize(c->stream), "collect");
     for (void *p : snaps)
         if (p) (void)hipFree(p);
-    for (void *p : {d64, r64, tmp, (void *)minus1})
+    for (void *p : {d64, r64, tmp, static_cast<void *>(minus1)})
         if (p) (void)hipFree(p);
     if (n_batches) *n_batches = int(c->lgnn_batches.size());
     return ok ? 0 : g_err;
@@ -714,6 +738,7 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
             z ^= z >> 31;
             std::swap(order[i - 1], order[size_t(z % i)]);
         }
+        const double lr = 0.5 * learning_rate * (1.0 + std::cos(M_PI * double(ep) / double(epochs)));
         double total = 0.0;
         int cur = -1;
         for (size_t i = 0; ok && i < order.size(); ++i) {
@@ -725,7 +750,7 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
             double L = 0.0;
             ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn train");
             ok = ok && hip_ok(c->precond->loss_grad(bt.r, bt.e, &L, c->stream), "lgnn train");
-            ok = ok && hip_ok(c->precond->adam_step(learning_rate, 1, c->stream), "lgnn train");
+            ok = ok && hip_ok(c->precond->adam_step(lr, 1, c->stream), "lgnn train");
             total += L;
         }
         if (loss_per_epoch) loss_per_epoch[ep] = total / (double(order.size()) * c->R);

@@ -23,6 +23,7 @@ struct SolverOptions {
     int inner_prec = kDouble;  // kDouble: plain FP64 BiCGStab; else reliable-update mixed precision
     double delta = 0.1;        // reliable update when ||r|| < delta * max ||r|| since last update
     int use_precond = 0;       // apply the learned right preconditioner (inner solve only)
+    double diverge = 1e6;      // stop (not converged) once a relative residual estimate exceeds this
     int verbose = 0;
 };
 
@@ -31,6 +32,7 @@ struct SolveResult {
     int iterations = 0;
     int reliable_updates = 0;
     int restarts = 0;
+    int diverged = 0;                 // stopped because a residual estimate exceeded SolverOptions::diverge
     double true_rel_residual = -1.0;  // max_r ||b - M x|| / ||b|| over the full lattice
     std::vector<double> rel_residual;  // per RHS, odd Schur system, last estimate
 };

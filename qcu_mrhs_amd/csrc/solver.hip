@@ -297,6 +297,10 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
             set_error(7, "BiCGStab breakdown (non-finite residual) at iteration %d", it + 1);
             return hipErrorUnknown;
         }
+        if (*std::max_element(res.rel_residual.begin(), res.rel_residual.end()) > opt.diverge) {
+            res.diverged = 1;
+            break;
+        }
         if (!mixed) {
             if (conv_est) {
                 converged = true;

@@ -97,16 +97,20 @@ def loss_grad(inp, target, channels):
     return loss.value, g1, g2
 
 
-def collect(max_iteration, tol, n_snapshots):
+def collect(max_iteration, tol, n_snapshots, n_random=0, seed=0):
     """Solve the queued systems (plain FP64) and keep n_snapshots
-    (error, residual) pairs per RHS; returns the number of stored batches."""
+    (error, residual) pairs per RHS from the solver's iterates plus n_random
+    pairs (Gaussian error, its residual); returns the number of stored batches."""
     nb = ctypes.c_int(0)
-    api.check(api.lib().qcuLgnnCollect(int(max_iteration), float(tol), int(n_snapshots), ctypes.byref(nb)))
+    api.check(api.lib().qcuLgnnCollect(int(max_iteration), float(tol), int(n_snapshots), int(n_random),
+                                       int(seed) & 0xFFFFFFFF, ctypes.byref(nb)))
     return nb.value
 
 
 def train(epochs, learning_rate, seed=0):
-    """Adam over the stored batches; returns the mean loss per pair per epoch."""
+    """Adam over the stored batches (cosine learning-rate decay from
+    ``learning_rate`` to 0 over the epochs); returns the mean loss per pair
+    per epoch."""
     losses = np.zeros(int(epochs), dtype=np.float64)
     api.check(api.lib().qcuLgnnTrain(int(epochs), float(learning_rate), int(seed) & 0xFFFFFFFF, _dptr(losses)))
     return losses
