@@ -160,6 +160,34 @@ int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int n_random,
 int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_per_epoch);
 int qcuLgnnClearData(void);
 
+/* ---- batched uneven right-hand-side groups (no reference counterpart) --
+ * Solves M x = b for up to 8 groups of sources of different sizes on the
+ * loaded gauge field in one batch: every step of the iteration is one launch
+ * over all groups and the inner iteration is replayed from a captured HIP
+ * graph; groups finish independently (device-side convergence flags).
+ * Precision: inner BiCGStab in the inner precision of qcuSetSolverOptions /
+ * initGridSize (FP32 storage recommended) inside an FP64 defect-correction
+ * loop, so every group reaches the FP64 tolerance `tol`.  Pointers are device
+ * full-lattice fermions in the user precision of initGridSize; the m_rhs of
+ * initGridSize does not limit the group sizes (1..128 each). */
+typedef struct QcuRhsGroup_s {
+    int m_rhs;
+    void **fermion_out;  /* m_rhs device pointers: solutions x */
+    void **fermion_in;   /* m_rhs device pointers: sources b   */
+} QcuRhsGroup;
+
+typedef struct QcuGroupInfo_s {
+    int converged;             /* every RHS of the group met tol (FP64 residual) */
+    int iterations;            /* inner iterations summed over outer steps        */
+    int outer_steps;           /* FP64 defect-correction steps                    */
+    int sites_per_wave;        /* dslash launch shape chosen by the autotuner     */
+    double true_rel_residual;  /* max_r ||b_r - M x_r|| / ||b_r||, FP64           */
+    double seconds;            /* solve start -> this group's convergence          */
+    double batch_seconds;      /* wall time of the whole call                     */
+} QcuGroupInfo;
+
+int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, double tol, QcuGroupInfo *info);
+
 /* Device-memory plumbing for callers that have no other GPU runtime. */
 void *qcuDeviceMalloc(size_t bytes);
 int qcuDeviceFree(void *ptr);

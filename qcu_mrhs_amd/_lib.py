@@ -38,6 +38,26 @@ class QcuSolveInfo(ctypes.Structure):
     ]
 
 
+class QcuRhsGroup(ctypes.Structure):
+    _fields_ = [
+        ("m_rhs", ctypes.c_int),
+        ("fermion_out", ctypes.POINTER(ctypes.c_void_p)),
+        ("fermion_in", ctypes.POINTER(ctypes.c_void_p)),
+    ]
+
+
+class QcuGroupInfo(ctypes.Structure):
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("outer_steps", ctypes.c_int),
+        ("sites_per_wave", ctypes.c_int),
+        ("true_rel_residual", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+        ("batch_seconds", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -88,6 +108,7 @@ def load() -> ctypes.CDLL:
         "qcuDslashAutotune": ([ip4, i, i, i, i, ip4, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i, ip4], i),
         "qcuDslashSetTuned": ([ip4, i, i, i, i, i, i, i], i),
+        "qcuInvertGroups": ([i, ctypes.POINTER(QcuRhsGroup), i, d, ctypes.POINTER(QcuGroupInfo)], i),
         "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnApply": ([vp, vp, i], i),

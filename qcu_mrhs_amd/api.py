@@ -16,13 +16,13 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import PRECISION, QcuGrid, QcuParam, QcuSolveInfo
+from ._lib import PRECISION, QcuGrid, QcuGroupInfo, QcuParam, QcuRhsGroup, QcuSolveInfo
 
 __all__ = [
     "QcuError", "DeviceArray", "PRECISION", "initGridSize", "pushBackFermions", "loadQcuGauge",
     "getDslash", "start_dslash", "mat_Qcu", "qcuInvert", "qcuInvertEx", "finalizeQcu",
     "gauge_eo_precondition", "gauge_reverse_eo_precondition", "read_gauge_from_file",
-    "set_solver_options", "dslash_mrhs", "to_storage", "from_storage", "synchronize",
+    "set_solver_options", "dslash_mrhs", "to_storage", "from_storage", "synchronize", "invert_groups",
 ]
 
 
@@ -199,6 +199,32 @@ def qcuInvertEx(max_iteration, tol, raise_on_nonconvergence=False):
     if code == 6:
         lib().qcuClearError()
     return res
+
+
+def invert_groups(groups, max_iteration, tol, raise_on_nonconvergence=False):
+    """Batched solve of uneven RHS groups (qcuInvertGroups).  ``groups`` is a
+    list of (xs, bs) pairs of equal-length lists of device buffers (full-lattice
+    fermions); returns one dict per group (converged, iterations, outer_steps,
+    sites_per_wave, true_rel_residual, seconds, batch_seconds)."""
+    n = len(groups)
+    arr = (QcuRhsGroup * max(n, 1))()
+    keep = []
+    for k, (xs, bs) in enumerate(groups):
+        if len(xs) != len(bs):
+            raise ValueError("each group needs as many solutions as sources")
+        outs = (ctypes.c_void_p * max(len(xs), 1))(*[_p(x) for x in xs])
+        ins = (ctypes.c_void_p * max(len(bs), 1))(*[_p(b) for b in bs])
+        keep += [outs, ins]
+        arr[k].m_rhs = len(xs)
+        arr[k].fermion_out = ctypes.cast(outs, ctypes.POINTER(ctypes.c_void_p))
+        arr[k].fermion_in = ctypes.cast(ins, ctypes.POINTER(ctypes.c_void_p))
+    info = (QcuGroupInfo * max(n, 1))()
+    code = lib().qcuInvertGroups(n, arr, int(max_iteration), float(tol), info)
+    if code and (code != 6 or raise_on_nonconvergence):
+        check(code)
+    if code == 6:
+        lib().qcuClearError()
+    return [{k: getattr(info[g], k) for k, _ in QcuGroupInfo._fields_} for g in range(n)]
 
 
 def set_solver_options(inner_precision=2, delta=0.1, use_preconditioner=False):

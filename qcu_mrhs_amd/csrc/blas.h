@@ -95,4 +95,44 @@ hipError_t launch_step_p(void *p, const void *r, const void *v, const C64 *st, l
 // set every slot to the values needed at the start of a solve
 hipError_t launch_scalar_fill(C64 *dst, C64 value, int count, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Batched groups (SegTab layout, qcu_common.h): one launch covers every group
+// (grid.y = group); groups whose done flag is set are skipped.  Scalars use
+// slot stride tab.Rtot.  Reductions keep partials per group and one
+// completion counter per (slot, group).
+// ---------------------------------------------------------------------------
+struct SegWork {
+    double *partials = nullptr;        // [G][max_blocks][Rmax][4]
+    unsigned int *counters = nullptr;  // [kReduceSlots][kMaxGroups]
+    int max_blocks = 0, G = 0, Rmax = 0;
+};
+hipError_t seg_work_alloc(SegWork &w, int G, int Rmax);
+void seg_work_free(SegWork &w);
+hipError_t seg_dotc(C64 *res, const void *a, const void *b, int prec, const SegTab &tab, SegWork &w, int slot,
+                    hipStream_t s);
+hipError_t seg_norm(double *res, const void *a, int prec, bool take_sqrt, const SegTab &tab, SegWork &w, int slot,
+                    hipStream_t s);
+hipError_t seg_step_alpha(C64 *st, const void *r0, const void *v, int prec, const SegTab &tab, SegWork &w,
+                          hipStream_t s);
+hipError_t seg_step_s(void *sv, const void *r, const void *v, const C64 *st, int prec, const SegTab &tab,
+                      hipStream_t s);
+hipError_t seg_step_omega(C64 *st, const void *t, const void *sv, int prec, const SegTab &tab, SegWork &w,
+                          hipStream_t s);
+// x += alpha p + omega s ; r = s - omega t ; ||r||^2 ; <r0, r> ; beta ; then per group:
+// iters[g] += 1 and done[g] = 1 once every RHS has ||r||^2 <= st[kBnorm2]
+hipError_t seg_step_xr(void *x, void *r, const void *p, const void *sv, const void *t, const void *r0, C64 *st,
+                       int *done, int *iters, int prec, const SegTab &tab, SegWork &w, hipStream_t s);
+// p = r + beta (p - omega v)
+hipError_t seg_step_p(void *p, const void *r, const void *v, const C64 *st, int prec, const SegTab &tab,
+                      hipStream_t s);
+// res = x - a[rhs] y
+hipError_t seg_xsay(void *res, const void *x, const void *y, const C64 *a, int prec, const SegTab &tab,
+                    hipStream_t s);
+// dst = scale[rhs] src (scale may be null)
+hipError_t seg_convert(void *dst, int dst_prec, const void *src, int src_prec, const double *scale,
+                       const SegTab &tab, hipStream_t s);
+// xhi (FP64) += scale[rhs] xlo
+hipError_t seg_accum(void *xhi, const void *xlo, int lo_prec, const double *scale, const SegTab &tab,
+                     hipStream_t s);
+
 }  // namespace qcu

@@ -69,10 +69,10 @@ __global__ __launch_bounds__(256) void k_axpy(CS *__restrict__ res, const CS *__
 }
 
 // ------------------------------------------------------------- reduction
+// Returns true in the block that ran the finisher (the last one to finish).
 template <class Op>
-__global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
-                                                          double *__restrict__ partials,
-                                                          unsigned int *__restrict__ counter) {
+__device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double *__restrict__ partials,
+                                            unsigned int *__restrict__ counter) {
     constexpr int NA = Op::NA;
     __shared__ double buf[NA][kReduceThreads];
     __shared__ bool am_last;
@@ -84,7 +84,13 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
     const bool active = rhs_lane(R, g, stride, rhs);
     if (active) {
         auto loc = op.local(rhs);
-        for (long e = g; e < N; e += stride) op.elem(e, loc, acc);
+        long e = g;
+        // two entries per trip keep more loads in flight (same summation order as one per trip)
+        for (; e + stride < N; e += 2 * stride) {
+            op.elem(e, loc, acc);
+            op.elem(e + stride, loc, acc);
+        }
+        if (e < N) op.elem(e, loc, acc);
     }
 #pragma unroll
     for (int k = 0; k < NA; ++k) buf[k][threadIdx.x] = acc[k];
@@ -108,7 +114,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
         am_last = (prev == gridDim.x - 1);
     }
     __syncthreads();
-    if (!am_last) return;
+    if (!am_last) return false;
     __threadfence();
     // last block: sum partials in block order
     const int t = threadIdx.x;
@@ -134,6 +140,14 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
         op.finish(t, tot);
     }
     if (t == 0) *counter = 0u;
+    return true;
+}
+
+template <class Op>
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
+                                                          double *__restrict__ partials,
+                                                          unsigned int *__restrict__ counter) {
+    (void)reduce_body(op, N, R, partials, counter);
 }
 
 template <typename CS>
@@ -149,6 +163,7 @@ struct DotOp {
         acc[1] = fma(u.re, v.im, fma(-u.im, v.re, acc[1]));
     }
     __device__ void finish(int r, const double (&t)[NA]) const { res[r] = {t[0], t[1]}; }
+    __device__ DotOp shifted(long off, int base) const { return {a + off, b + off, res + base}; }
 };
 
 template <typename CS>
@@ -166,6 +181,7 @@ struct NormOp {
     __device__ void finish(int r, const double (&t)[NA]) const {
         res[r] = take_sqrt ? sqrt(t[0]) : t[0];
     }
+    __device__ NormOp shifted(long off, int base) const { return {a + off, res + base, take_sqrt}; }
 };
 
 template <typename CS>
@@ -188,6 +204,7 @@ struct AlphaOp {  // <r0, v>, alpha = rho / <r0, v>
         st[kAlpha * R + r] = (r0v.re == 0.0 && r0v.im == 0.0) ? C64{0.0, 0.0}
                                                                 : cdiv(st[kRho * R + r], r0v);
     }
+    __device__ AlphaOp shifted(long off, int base) const { return {r0 + off, v + off, st + base, R}; }
 };
 
 template <typename CS>
@@ -210,6 +227,7 @@ struct OmegaOp {  // <t,s>, <t,t>, omega = <t,s>/<t,t>
         st[kTt * R + r] = t2;
         st[kOmega * R + r] = t2.re == 0.0 ? C64{0.0, 0.0} : cdiv(ts, t2);
     }
+    __device__ OmegaOp shifted(long off, int base) const { return {t + off, s + off, st + base, R}; }
 };
 
 template <typename CS, typename T>
@@ -250,6 +268,9 @@ struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
         st[kBeta * R + rr] = beta;
         st[kTmpA * R + rr] = st[kRho * R + rr];  // rho of this iteration (reliable updates)
         st[kRho * R + rr] = rho_new;
+    }
+    __device__ XROp shifted(long off, int base) const {
+        return {x + off, r + off, p + off, shat + off, s + off, t + off, r0 + off, st + base, R};
     }
 };
 
@@ -298,6 +319,129 @@ __global__ void k_fill(C64 *dst, C64 v, int count) {
     if (i < count) dst[i] = v;
 }
 
+// ------------------------------------------------------------ batched groups
+// blockIdx.y = group; the group's vectors start n * base entries into each
+// batch vector and its scalars `base` entries into each slot.
+__device__ __forceinline__ bool seg_group(const SegTab &tab, int &grp, int &R, long &off, int &base) {
+    grp = blockIdx.y;
+    if (tab.done && tab.done[grp]) return false;
+    R = tab.R[grp];
+    base = tab.base[grp];
+    off = tab.n * base;
+    return true;
+}
+
+// Reduction over every group; with GROUP_DONE the finishing block of a group
+// also counts the iteration and raises the group's done flag once every RHS
+// has ||r||^2 <= st[kBnorm2] (the squared stopping threshold).
+template <class Op, bool GROUP_DONE>
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_seg(Op op, SegTab tab, double *__restrict__ partials,
+                                                              unsigned int *__restrict__ counters, int max_blocks,
+                                                              const C64 *st, int *done, int *iters) {
+    int grp, R, base;
+    long off;
+    if (!seg_group(tab, grp, R, off, base)) return;
+    const bool last = reduce_body(op.shifted(off, base), tab.n * R, R,
+                                  partials + size_t(grp) * max_blocks * tab.Rmax * 4, counters + grp);
+    if constexpr (GROUP_DONE) {
+        if (last) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                bool all = true;
+                for (int r = 0; r < R; ++r)
+                    all = all && st[kRnorm2 * tab.Rtot + base + r].re <= st[kBnorm2 * tab.Rtot + base + r].re;
+                iters[grp] += 1;
+                if (all) done[grp] = 1;
+            }
+        }
+    }
+}
+
+template <typename CS, typename T>
+__global__ __launch_bounds__(256) void k_step_s_seg(CS *sv, const CS *r, const CS *v, const C64 *st, SegTab tab) {
+    int grp, R, base;
+    long off, g, stride;
+    int rhs;
+    if (!seg_group(tab, grp, R, off, base) || !rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> al = tc<T>(st[kAlpha * tab.Rtot + base + rhs]);
+    const long N = tab.n * R;
+    sv += off;
+    r += off;
+    v += off;
+    for (long e = g; e < N; e += stride) store_cx(sv + e, load_cx<T>(r + e) - al * load_cx<T>(v + e));
+}
+
+template <typename CS, typename T>
+__global__ __launch_bounds__(256) void k_step_p_seg(CS *p, const CS *r, const CS *v, const C64 *st, SegTab tab) {
+    int grp, R, base;
+    long off, g, stride;
+    int rhs;
+    if (!seg_group(tab, grp, R, off, base) || !rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> om = tc<T>(st[kOmega * tab.Rtot + base + rhs]);
+    const Cx<T> be = tc<T>(st[kBeta * tab.Rtot + base + rhs]);
+    const long N = tab.n * R;
+    p += off;
+    r += off;
+    v += off;
+    for (long e = g; e < N; e += stride) {
+        const Cx<T> q = load_cx<T>(p + e) - om * load_cx<T>(v + e);
+        store_cx(p + e, load_cx<T>(r + e) + be * q);
+    }
+}
+
+// dst = scale[rhs] * src (scale may be null)
+template <typename CD, typename CS>
+__global__ __launch_bounds__(256) void k_convert_seg(CD *dst, const CS *src, const double *scale, SegTab tab) {
+    int grp, R, base;
+    long off, g, stride;
+    int rhs;
+    if (!seg_group(tab, grp, R, off, base) || !rhs_lane(R, g, stride, rhs)) return;
+    const double sc = scale ? scale[base + rhs] : 1.0;
+    const long N = tab.n * R;
+    dst += off;
+    src += off;
+    for (long e = g; e < N; e += stride) {
+        Cx<double> v = load_cx<double>(src + e);
+        if (scale) v = sc * v;
+        store_cx(dst + e, v);
+    }
+}
+
+// res = x - a[rhs] y
+template <typename CS, typename T>
+__global__ __launch_bounds__(256) void k_xsay_seg(CS *res, const CS *x, const CS *y, const C64 *a, SegTab tab) {
+    int grp, R, base;
+    long off, g, stride;
+    int rhs;
+    if (!seg_group(tab, grp, R, off, base) || !rhs_lane(R, g, stride, rhs)) return;
+    const Cx<T> ar = tc<T>(a[base + rhs]);
+    const long N = tab.n * R;
+    res += off;
+    x += off;
+    y += off;
+    for (long e = g; e < N; e += stride) store_cx(res + e, load_cx<T>(x + e) - ar * load_cx<T>(y + e));
+}
+
+// xhi += scale[rhs] * xlo (FP64 accumulation)
+template <typename CS>
+__global__ __launch_bounds__(256) void k_accum_seg(Cx<double> *xhi, const CS *xlo, const double *scale, SegTab tab) {
+    int grp, R, base;
+    long off, g, stride;
+    int rhs;
+    if (!seg_group(tab, grp, R, off, base) || !rhs_lane(R, g, stride, rhs)) return;
+    const double sc = scale[base + rhs];
+    const long N = tab.n * R;
+    xhi += off;
+    xlo += off;
+    for (long e = g; e < N; e += stride) {
+        const Cx<double> v = load_cx<double>(xlo + e);
+        Cx<double> h = xhi[e];
+        h.re = fma(sc, v.re, h.re);
+        h.im = fma(sc, v.im, h.im);
+        xhi[e] = h;
+    }
+}
+
 inline int stream_blocks(long N, int R) {
     const int per_block = (256 / R) * R;
     long want = (N + per_block - 1) / per_block;
@@ -326,7 +470,7 @@ int reduce_blocks(long n, int R, int max_blocks) {
 }
 
 hipError_t reduce_work_alloc(ReduceWork &w, int R) {
-    w.max_blocks = 1024;
+    w.max_blocks = 4096;  // 16 workgroups per CU: enough loads in flight for 4-byte storage
     QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R));
     QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots));
     QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots));
@@ -455,6 +599,162 @@ hipError_t launch_step_p(void *p, const void *r, const void *v, const C64 *st, l
                            static_cast<const CS *>(r), static_cast<const CS *>(v), st, N, R);
         return hipGetLastError();
     })
+}
+
+// ------------------------------------------------------------ batched groups
+hipError_t seg_work_alloc(SegWork &w, int G, int Rmax) {
+    seg_work_free(w);
+    w.max_blocks = 2048;
+    w.G = G;
+    w.Rmax = Rmax;
+    QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * Rmax * G));
+    QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots * kMaxGroups));
+    QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots * kMaxGroups));
+    QCU_HIP_TRY(hipDeviceSynchronize());
+    return hipSuccess;
+}
+
+void seg_work_free(SegWork &w) {
+    if (w.partials) (void)hipFree(w.partials);
+    if (w.counters) (void)hipFree(w.counters);
+    w = SegWork{};
+}
+
+namespace {
+
+bool seg_ok(const SegTab &tab, const SegWork *w) {
+    if (tab.G < 1 || tab.G > kMaxGroups || tab.n <= 0) return false;
+    for (int k = 0; k < tab.G; ++k)
+        if (tab.R[k] < 1 || tab.R[k] > tab.Rmax || tab.R[k] > kReduceThreads) return false;
+    return !w || (w->partials && tab.G <= w->G && tab.Rmax <= w->Rmax);
+}
+
+template <class Op, bool GD = false>
+hipError_t run_seg_reduce(const Op &op, const SegTab &tab, SegWork &w, int slot, hipStream_t s,
+                          const C64 *st = nullptr, int *done = nullptr, int *iters = nullptr) {
+    if (!seg_ok(tab, &w) || slot < 0 || slot >= kReduceSlots) return hipErrorInvalidValue;
+    const int nb = reduce_blocks(tab.n, tab.Rmax, w.max_blocks);
+    hipLaunchKernelGGL((k_reduce_seg<Op, GD>), dim3(nb, tab.G), dim3(kReduceThreads), 0, s, op, tab, w.partials,
+                       w.counters + slot * kMaxGroups, w.max_blocks, st, done, iters);
+    return hipGetLastError();
+}
+
+dim3 seg_grid(const SegTab &tab) { return dim3(stream_blocks(tab.n * tab.Rmax, tab.Rmax), tab.G); }
+
+}  // namespace
+
+hipError_t seg_dotc(C64 *res, const void *a, const void *b, int prec, const SegTab &tab, SegWork &w, int slot,
+                    hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        DotOp<CS> op{static_cast<const CS *>(a), static_cast<const CS *>(b), res};
+        return run_seg_reduce(op, tab, w, slot, s);
+    })
+}
+
+hipError_t seg_norm(double *res, const void *a, int prec, bool take_sqrt, const SegTab &tab, SegWork &w, int slot,
+                    hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        NormOp<CS> op{static_cast<const CS *>(a), res, take_sqrt};
+        return run_seg_reduce(op, tab, w, slot, s);
+    })
+}
+
+hipError_t seg_step_alpha(C64 *st, const void *r0, const void *v, int prec, const SegTab &tab, SegWork &w,
+                          hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        AlphaOp<CS> op{static_cast<const CS *>(r0), static_cast<const CS *>(v), st, tab.Rtot};
+        return run_seg_reduce(op, tab, w, 0, s);
+    })
+}
+
+hipError_t seg_step_omega(C64 *st, const void *t, const void *sv, int prec, const SegTab &tab, SegWork &w,
+                          hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        OmegaOp<CS> op{static_cast<const CS *>(t), static_cast<const CS *>(sv), st, tab.Rtot};
+        return run_seg_reduce(op, tab, w, 1, s);
+    })
+}
+
+hipError_t seg_step_xr(void *x, void *r, const void *p, const void *sv, const void *t, const void *r0, C64 *st,
+                       int *done, int *iters, int prec, const SegTab &tab, SegWork &w, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        XROp<CS, T> op{static_cast<CS *>(x), static_cast<CS *>(r), static_cast<const CS *>(p),
+                       static_cast<const CS *>(sv), static_cast<const CS *>(sv), static_cast<const CS *>(t),
+                       static_cast<const CS *>(r0), st, tab.Rtot};
+        return run_seg_reduce<decltype(op), true>(op, tab, w, 2, s, st, done, iters);
+    })
+}
+
+hipError_t seg_step_s(void *sv, const void *r, const void *v, const C64 *st, int prec, const SegTab &tab,
+                      hipStream_t s) {
+    if (!seg_ok(tab, nullptr)) return hipErrorInvalidValue;
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        hipLaunchKernelGGL((k_step_s_seg<CS, T>), seg_grid(tab), dim3(256), 0, s, static_cast<CS *>(sv),
+                           static_cast<const CS *>(r), static_cast<const CS *>(v), st, tab);
+        return hipGetLastError();
+    })
+}
+
+hipError_t seg_step_p(void *p, const void *r, const void *v, const C64 *st, int prec, const SegTab &tab,
+                      hipStream_t s) {
+    if (!seg_ok(tab, nullptr)) return hipErrorInvalidValue;
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        hipLaunchKernelGGL((k_step_p_seg<CS, T>), seg_grid(tab), dim3(256), 0, s, static_cast<CS *>(p),
+                           static_cast<const CS *>(r), static_cast<const CS *>(v), st, tab);
+        return hipGetLastError();
+    })
+}
+
+hipError_t seg_xsay(void *res, const void *x, const void *y, const C64 *a, int prec, const SegTab &tab,
+                    hipStream_t s) {
+    if (!seg_ok(tab, nullptr)) return hipErrorInvalidValue;
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        using T = typename PrecT<P>::T;
+        hipLaunchKernelGGL((k_xsay_seg<CS, T>), seg_grid(tab), dim3(256), 0, s, static_cast<CS *>(res),
+                           static_cast<const CS *>(x), static_cast<const CS *>(y), a, tab);
+        return hipGetLastError();
+    })
+}
+
+hipError_t seg_convert(void *dst, int dst_prec, const void *src, int src_prec, const double *scale,
+                       const SegTab &tab, hipStream_t s) {
+    if (!seg_ok(tab, nullptr)) return hipErrorInvalidValue;
+    QCU_PREC_DISPATCH(dst_prec, {
+        using CD = typename CxStore<typename PrecT<P>::S>::type;
+        switch (src_prec) {
+            case kDouble: hipLaunchKernelGGL((k_convert_seg<CD, Cx<double>>), seg_grid(tab), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Cx<double> *>(src), scale, tab); break;
+            case kSingle: hipLaunchKernelGGL((k_convert_seg<CD, Cx<float>>), seg_grid(tab), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Cx<float> *>(src), scale, tab); break;
+            case kHalf: hipLaunchKernelGGL((k_convert_seg<CD, Half2Store>), seg_grid(tab), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Half2Store *>(src), scale, tab); break;
+            case kBF16: hipLaunchKernelGGL((k_convert_seg<CD, Bf162Store>), seg_grid(tab), dim3(256), 0, s, static_cast<CD *>(dst), static_cast<const Bf162Store *>(src), scale, tab); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    })
+}
+
+hipError_t seg_accum(void *xhi, const void *xlo, int lo_prec, const double *scale, const SegTab &tab,
+                     hipStream_t s) {
+    if (!seg_ok(tab, nullptr) || !scale) return hipErrorInvalidValue;
+    auto *h = static_cast<Cx<double> *>(xhi);
+    switch (lo_prec) {
+        case kDouble: hipLaunchKernelGGL((k_accum_seg<Cx<double>>), seg_grid(tab), dim3(256), 0, s, h, static_cast<const Cx<double> *>(xlo), scale, tab); break;
+        case kSingle: hipLaunchKernelGGL((k_accum_seg<Cx<float>>), seg_grid(tab), dim3(256), 0, s, h, static_cast<const Cx<float> *>(xlo), scale, tab); break;
+        case kHalf: hipLaunchKernelGGL((k_accum_seg<Half2Store>), seg_grid(tab), dim3(256), 0, s, h, static_cast<const Half2Store *>(xlo), scale, tab); break;
+        case kBF16: hipLaunchKernelGGL((k_accum_seg<Bf162Store>), seg_grid(tab), dim3(256), 0, s, h, static_cast<const Bf162Store *>(xlo), scale, tab); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_scalar_fill(C64 *dst, C64 value, int count, hipStream_t s) {

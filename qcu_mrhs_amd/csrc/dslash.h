@@ -50,4 +50,10 @@ hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStrea
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 bool dslash_mfma_supported(int Nc, int prec);
 
+// One launch of the wave-per-site kernel over every group of a batch (grid.y =
+// group): out/in/xpay are batch half vectors (SegTab layout), a.R is ignored,
+// the group's sites per wavefront come from tab.spw, the traversal tile from
+// (by, bz).  Groups whose done flag is set are skipped.
+hipError_t launch_dslash_wave_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream);
+
 }  // namespace qcu

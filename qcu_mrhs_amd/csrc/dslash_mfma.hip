@@ -99,13 +99,21 @@ __device__ __forceinline__ uint32_t apply(const WordOp &o, uint32_t w) {
     return __builtin_amdgcn_perm(w, w, o.sel) ^ o.mask;
 }
 
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), 0);
+}
+
+// One wavefront = `spw` consecutive traversal positions x one 16-RHS chunk
+// (blockIdx.y).  Per site the wavefront needs one scalar row of the walk
+// table (site + 8 neighbours), prefetched one site ahead, so the only exposed
+// memory latency per site is that of its link and spinor loads, which are
+// all issued back to back before any of them is waited for.
 template <typename S, int NC, bool DAG>
 __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ gauge,
                                                     const uint32_t *__restrict__ xpay, float coef, int Vh, int R,
-                                                    int parity, const int *__restrict__ order,
-                                                    const int *__restrict__ nbrtab, int spw, unsigned vec_bytes,
-                                                    unsigned gauge_bytes) {
+                                                    int parity, const int *__restrict__ walk, int spw,
+                                                    unsigned vec_bytes, unsigned gauge_bytes) {
     using P = Pk<S>;
     constexpr int KH = (2 * NC <= 8) ? 8 : 16;  // k entries per hop
     constexpr int HPS = 32 / KH;                 // hops per MFMA
@@ -118,13 +126,16 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
     const int m = lane & 15;
     const int grp = lane >> 4;
     const int csub = (grp % GPH) * 4;           // first input colour of this lane's k entries
+    const bool gb0 = (grp & 1) != 0, gb1 = (grp & 2) != 0;
     const int cb = m >> 1;                      // output colour of B / D column m
     const bool bre = (m & 1) == 0;
-    const bool bvalid = cb < NC;
-    const int cbc = bvalid ? cb : NC - 1;       // clamped (column never stored)
+    const int cbc = cb < NC ? cb : NC - 1;      // clamped (column never stored)
     const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int r0 = 16 * int(blockIdx.y);
+    const int r = r0 + m;
     const unsigned row_bytes = unsigned(R) * 4u;            // one (spin, colour) row of R words
     const unsigned site_bytes = 4u * NC * row_bytes;
+    const unsigned rcb = unsigned(min(r, R - 1)) * 4u;      // rows beyond R are never stored
     const unsigned ll = unsigned(NC) * NC;
     const auto r_in = rsrc(in, vec_bytes);
     const auto r_g = rsrc(gauge, gauge_bytes);
@@ -132,8 +143,8 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
     const auto r_x = rsrc(xpay, xpay ? vec_bytes : 0u);
 
     // per-lane, per-step hop constants
-    int hop_s[STEPS], mu_s[STEPS];
-    bool fwd_s[STEPS], m02_s[STEPS];
+    int mu_s[STEPS];
+    bool fwd_s[STEPS];
     WordOp p0_s[STEPS], p1_s[STEPS], r0_s[STEPS], r1_s[STEPS], bop_s[STEPS];
 #pragma unroll
     for (int st = 0; st < STEPS; ++st) {
@@ -141,10 +152,8 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         const int mu = h >> 1;
         const bool fwd = (h & 1) == 0;
         const HopInfo hi = hop_info(mu, fwd, DAG);
-        hop_s[st] = h;
         mu_s[st] = mu;
         fwd_s[st] = fwd;
-        m02_s[st] = hi.m0_is_2;
         p0_s[st] = word_op(hi.kp0);
         p1_s[st] = word_op(hi.kp1);
         r0_s[st] = word_op(hi.kr0);
@@ -155,94 +164,115 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         bop_s[st].mask = fwd ? (bre ? 0x80000000u : 0u) : (bre ? 0u : 0x00008000u);
     }
 
-    for (int j = 0; j < spw; ++j) {
-        const int q = (b * spw + j) * 4 + wv;
-        if (q >= Vh) break;  // uniform
-        const int site = order[q];
-        const int *nrow = nbrtab + (long(parity) * Vh + site) * 8;
+    const int *wtab = walk + long(parity) * Vh * 9;
+    const int q0 = (b * 4 + wv) * spw;
+    if (q0 >= Vh) return;  // uniform
+    int cur[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cur[k] = wtab[long(q0) * 9 + k];
 
-        unsigned voff_a[STEPS], voff_b0[STEPS], voff_b1[STEPS];
-        u32x4 bfrag[STEPS];
+    for (int j = 0; j < spw; ++j) {
+        const int q = q0 + j;
+        if (q >= Vh) break;  // uniform
+        const bool more = (j + 1 < spw) && (q + 1 < Vh);
+        const int site = cur[0];
+
+        unsigned voff_a[STEPS], voff_b0[STEPS], voff_b1[STEPS], gaddr[STEPS], gstep[STEPS];
 #pragma unroll
         for (int st = 0; st < STEPS; ++st) {
-            const int nb = nrow[hop_s[st]];
-            const unsigned va = unsigned(nb) * site_bytes;
-            // spin m(0) is 3 for x, y hops and 2 for z, t hops; spin 2 row is the base
+            // this lane's hop: two-level select of the scalar neighbour row (no branches)
+            int nb;
+            if constexpr (HPS == 4) {
+                const int lo = gb0 ? cur[2 + 4 * st] : cur[1 + 4 * st];
+                const int hi = gb0 ? cur[4 + 4 * st] : cur[3 + 4 * st];
+                nb = gb1 ? hi : lo;
+            } else {
+                nb = gb1 ? cur[2 + 2 * st] : cur[1 + 2 * st];
+            }
+            const unsigned va = unsigned(nb) * site_bytes + rcb + unsigned(csub) * row_bytes;
+            // spin m(0) is 3 for x, y hops and 2 for z, t hops: lower-spin rows of the projection
+            const bool m02 = (st * HPS) >= 4;
             voff_a[st] = va;
-            voff_b0[st] = va + (m02_s[st] ? 0u : unsigned(NC) * row_bytes);
-            voff_b1[st] = va + (m02_s[st] ? unsigned(NC) * row_bytes : 0u);
+            voff_b0[st] = va + (m02 ? 0u : unsigned(NC) * row_bytes);
+            voff_b1[st] = va + (m02 ? unsigned(NC) * row_bytes : 0u);
             const unsigned gl = fwd_s[st] ? (unsigned(mu_s[st]) * 2 + parity) * unsigned(Vh) + unsigned(site)
                                           : (unsigned(mu_s[st]) * 2 + 1 - parity) * unsigned(Vh) + unsigned(nb);
-            const unsigned gstride = fwd_s[st] ? 4u : 4u * NC;  // step of c' in bytes
-            const unsigned gbase = (gl * ll + unsigned(cbc) * (fwd_s[st] ? NC : 1)) * 4u + unsigned(csub) * gstride;
+            gstep[st] = fwd_s[st] ? 4u : 4u * NC;  // step of c' in bytes
+            gaddr[st] = (gl * ll + unsigned(cbc) * (fwd_s[st] ? NC : 1)) * 4u + unsigned(csub) * gstep[st];
+        }
+        // prefetch the next walk row only now, after the last use of this one
+        int nxt[9];
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) nxt[k] = wtab[long(q + 1) * 9 + k];
+        }
+        // every load of this site first: spinor rows, then links
+        uint32_t w0[STEPS][4], w1[STEPS][4], wb0[STEPS][4], wb1[STEPS][4], lw[STEPS][4];
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st)
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
-                uint32_t w = 0u;
-                if (cc < CPL) w = apply(bop_s[st], bload(r_g, gbase + unsigned(cc) * gstride, 0u));
-                bfrag[st][cc] = w;
+                if (cc < CPL) {
+                    const unsigned c1 = unsigned(cc) * row_bytes;
+                    w0[st][cc] = bload(r_in, voff_a[st] + c1, 0u);
+                    w1[st][cc] = bload(r_in, voff_a[st] + c1, unsigned(NC) * row_bytes);
+                    wb0[st][cc] = bload(r_in, voff_b0[st] + c1, 2u * NC * row_bytes);
+                    wb1[st][cc] = bload(r_in, voff_b1[st] + c1, 2u * NC * row_bytes);
+                } else {
+                    w0[st][cc] = w1[st][cc] = wb0[st][cc] = wb1[st][cc] = 0u;
+                }
+            }
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st)
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) lw[st][cc] = cc < CPL ? bload(r_g, gaddr[st] + unsigned(cc) * gstep[st], 0u) : 0u;
+
+        f32x4 d[4];
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st) {
+            const bool m02 = (st * HPS) >= 4;
+            u32x4 a[4], bf;
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                const uint32_t h0 = P::add(w0[st][cc], apply(p0_s[st], wb0[st][cc]));
+                const uint32_t h1 = P::add(w1[st][cc], apply(p1_s[st], wb1[st][cc]));
+                const uint32_t g0 = apply(r0_s[st], h0), g1 = apply(r1_s[st], h1);
+                a[0][cc] = h0;
+                a[1][cc] = h1;
+                a[2][cc] = m02 ? g0 : g1;
+                a[3][cc] = m02 ? g1 : g0;
+                bf[cc] = cc < CPL ? apply(bop_s[st], lw[st][cc]) : 0u;
+            }
+            // links as the A operand, spinors as B: D^T = B^T A^T, so the
+            // accumulator rows are (colour, re|im) and its column is this lane's RHS
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp) d[sp] = P::mma(bf, a[sp], d[sp]);
+        }
+        // lane (m, grp) holds rows 4*grp + e = (colour 2*grp + e/2, re|im) of RHS r
+        if (r < R) {
+            const unsigned obase = unsigned(site) * site_bytes + unsigned(r) * 4u + unsigned(2 * grp) * row_bytes;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                if (2 * grp + i < NC) {
+#pragma unroll
+                    for (int sp = 0; sp < 4; ++sp) {
+                        const unsigned so = unsigned(sp * NC + i) * row_bytes;
+                        float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
+                        if (xpay) {
+                            const uint32_t xw = bload(r_x, obase, so);
+                            vr += P::re(xw);
+                            vi += P::im(xw);
+                        }
+                        bstore(r_out, P::pack(vr, vi), obase, so);
+                    }
+                }
             }
         }
-
-        for (int r0 = 0; r0 < R; r0 += 16) {
-            const unsigned rcb = unsigned(min(r0 + m, R - 1)) * 4u;  // rows beyond R are never stored
-            f32x4 d[4];
+        if (more) {
 #pragma unroll
-            for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
-            // every neighbour word of this site first: independent loads
-            uint32_t w0[STEPS][4], w1[STEPS][4], wb0[STEPS][4], wb1[STEPS][4];
-#pragma unroll
-            for (int st = 0; st < STEPS; ++st)
-#pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    if (cc < CPL) {
-                        const unsigned c1 = unsigned(csub + cc) * row_bytes;
-                        w0[st][cc] = bload(r_in, voff_a[st] + rcb + c1, 0u);
-                        w1[st][cc] = bload(r_in, voff_a[st] + rcb + c1, unsigned(NC) * row_bytes);
-                        wb0[st][cc] = bload(r_in, voff_b0[st] + rcb + c1, 2u * NC * row_bytes);
-                        wb1[st][cc] = bload(r_in, voff_b1[st] + rcb + c1, 2u * NC * row_bytes);
-                    } else {
-                        w0[st][cc] = w1[st][cc] = wb0[st][cc] = wb1[st][cc] = 0u;
-                    }
-                }
-#pragma unroll
-            for (int st = 0; st < STEPS; ++st) {
-                u32x4 a[4];
-#pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    const uint32_t h0 = P::add(w0[st][cc], apply(p0_s[st], wb0[st][cc]));
-                    const uint32_t h1 = P::add(w1[st][cc], apply(p1_s[st], wb1[st][cc]));
-                    const uint32_t g0 = apply(r0_s[st], h0), g1 = apply(r1_s[st], h1);
-                    a[0][cc] = h0;
-                    a[1][cc] = h1;
-                    a[2][cc] = m02_s[st] ? g0 : g1;
-                    a[3][cc] = m02_s[st] ? g1 : g0;
-                }
-                // links as the A operand, spinors as B: D^T = B^T A^T, so the
-                // accumulator rows are (colour, re|im) and its column is this lane's RHS
-#pragma unroll
-                for (int sp = 0; sp < 4; ++sp) d[sp] = P::mma(bfrag[st], a[sp], d[sp]);
-            }
-            // lane (m, grp) holds rows 4*grp + e = (colour 2*grp + e/2, re|im) of RHS r0 + m
-            const int r = r0 + m;
-            if (r < R) {
-#pragma unroll
-                for (int sp = 0; sp < 4; ++sp) {
-#pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const int c = 2 * grp + i;
-                        if (c < NC) {
-                            const unsigned off = ((unsigned(site) * 4u + sp) * NC + unsigned(c)) * row_bytes + unsigned(r) * 4u;
-                            float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
-                            if (xpay) {
-                                const uint32_t xw = bload(r_x, off, 0u);
-                                vr += P::re(xw);
-                                vi += P::im(xw);
-                            }
-                            __builtin_amdgcn_raw_buffer_store_b32(int(P::pack(vr, vi)), r_out, int(off), 0, 0);
-                        }
-                    }
-                }
-            }
+            for (int k = 0; k < 9; ++k) cur[k] = nxt[k];
         }
     }
 }
@@ -251,7 +281,7 @@ template <typename S, int NC>
 hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(a.g, t.by, t.bz, tab, stream));
-    dim3 grid(div_up(a.g.Vh, 4 * t.spw)), block(256);
+    dim3 grid(div_up(a.g.Vh, 4 * t.spw), div_up(a.R, 16)), block(256);
     auto out = static_cast<uint32_t *>(a.out);
     auto in = static_cast<const uint32_t *>(a.in);
     auto gauge = static_cast<const uint32_t *>(a.gauge);
@@ -260,10 +290,10 @@ hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) 
     const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
     if (a.dagger)
         hipLaunchKernelGGL((k_dslash_mfma<S, NC, true>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
-                           a.g.Vh, a.R, a.parity, tab.order, tab.nbr, t.spw, vec_bytes, gauge_bytes);
+                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
     else
         hipLaunchKernelGGL((k_dslash_mfma<S, NC, false>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
-                           a.g.Vh, a.R, a.parity, tab.order, tab.nbr, t.spw, vec_bytes, gauge_bytes);
+                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
     return hipGetLastError();
 }
 

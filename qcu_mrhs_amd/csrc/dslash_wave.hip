@@ -10,6 +10,8 @@
 // row per (spin, colour).  Only a few sites per XCD are in flight, and the
 // traversal (tiles in y,z swept along t, contiguous per XCD) keeps every
 // neighbour reuse within the XCD's L2.
+#include <algorithm>
+
 #include "dslash_common.h"
 
 namespace qcu {
@@ -58,16 +60,17 @@ __device__ __forceinline__ void hop_rt(Cx<T> (&acc)[4][NC], const CS *__restrict
     }
 }
 
+// Workgroup body: 4 wavefronts x spw sites, starting at traversal position
+// 4 * spw * b.
 template <typename T, typename CS, int NC, bool DAG>
-__global__ __launch_bounds__(256) void k_dslash_wave(CS *__restrict__ out, const CS *__restrict__ in,
-                                                    const CS *__restrict__ gauge,
-                                                    const CS *__restrict__ xpay, T coef, Geom g, int R,
-                                                    int parity, int by, int bz, int spw) {
+__device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS *__restrict__ in,
+                                           const CS *__restrict__ gauge, const CS *__restrict__ xpay,
+                                           T coef, const Geom &g, int R, int parity, int by, int bz,
+                                           int spw) {
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int mu = lane >> 4;
     const int rl = lane & 15;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const long site_len = 4L * NC * R;
     const long link_len = long(NC) * NC;
     const long Vh = g.Vh;
@@ -133,6 +136,50 @@ __global__ __launch_bounds__(256) void k_dslash_wave(CS *__restrict__ out, const
     }
 }
 
+template <typename T, typename CS, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_wave(CS *__restrict__ out, const CS *__restrict__ in,
+                                                    const CS *__restrict__ gauge,
+                                                    const CS *__restrict__ xpay, T coef, Geom g, int R,
+                                                    int parity, int by, int bz, int spw) {
+    wave_block<T, CS, NC, DAG>(xcd_remap(blockIdx.x, gridDim.x), out, in, gauge, xpay, coef, g, R, parity,
+                               by, bz, spw);
+}
+
+// every group of a batch in one launch: blockIdx.y = group
+template <typename T, typename CS, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_wave_groups(CS *__restrict__ out, const CS *__restrict__ in,
+                                                           const CS *__restrict__ gauge,
+                                                           const CS *__restrict__ xpay, T coef, Geom g,
+                                                           int parity, int by, int bz, SegTab tab) {
+    const int grp = blockIdx.y;
+    if (tab.done && tab.done[grp]) return;
+    const int spw = tab.spw[grp];
+    const int nblk = (g.Vh + 4 * spw - 1) / (4 * spw);
+    if (int(blockIdx.x) >= nblk) return;
+    const long off = tab.n * tab.base[grp];
+    wave_block<T, CS, NC, DAG>(xcd_remap(blockIdx.x, nblk), out + off, in + off, gauge,
+                               xpay ? xpay + off : nullptr, coef, g, tab.R[grp], parity, by, bz, spw);
+}
+
+template <typename T, typename S, int NC>
+hipError_t launch_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream) {
+    using CS = typename CxStore<S>::type;
+    int nmax = 1;
+    for (int k = 0; k < tab.G; ++k) nmax = std::max(nmax, div_up(a.g.Vh, 4 * tab.spw[k]));
+    dim3 grid(nmax, tab.G), block(256);
+    auto out = static_cast<CS *>(a.out);
+    auto in = static_cast<const CS *>(a.in);
+    auto gauge = static_cast<const CS *>(a.gauge);
+    auto xp = static_cast<const CS *>(a.xpay);
+    if (a.dagger)
+        hipLaunchKernelGGL((k_dslash_wave_groups<T, CS, NC, true>), grid, block, 0, stream, out, in, gauge, xp,
+                           T(a.coef), a.g, a.parity, by, bz, tab);
+    else
+        hipLaunchKernelGGL((k_dslash_wave_groups<T, CS, NC, false>), grid, block, 0, stream, out, in, gauge, xp,
+                           T(a.coef), a.g, a.parity, by, bz, tab);
+    return hipGetLastError();
+}
+
 template <typename T, typename S, int NC>
 hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     using CS = typename CxStore<S>::type;
@@ -149,6 +196,17 @@ hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) 
         hipLaunchKernelGGL((k_dslash_wave<T, CS, NC, false>), grid, block, 0, stream, out, in, gauge, xp,
                            T(a.coef), a.g, a.R, a.parity, t.by, t.bz, t.spw);
     return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t dispatch_nc_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_groups<T, S, 2>(a, tab, by, bz, stream);
+        case 3: return launch_groups<T, S, 3>(a, tab, by, bz, stream);
+        case 4: return launch_groups<T, S, 4>(a, tab, by, bz, stream);
+        case 8: return launch_groups<T, S, 8>(a, tab, by, bz, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T, typename S>
@@ -171,6 +229,20 @@ hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStrea
         case kSingle: return dispatch_nc<float, float>(a, t, stream);
         case kHalf: return dispatch_nc<float, __half>(a, t, stream);
         case kBF16: return dispatch_nc<float, __hip_bfloat16>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_dslash_wave_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream) {
+    if (tab.G < 1 || tab.G > kMaxGroups || by <= 0 || bz <= 0 || a.g.L[1] % by || a.g.L[2] % bz)
+        return hipErrorInvalidValue;
+    for (int k = 0; k < tab.G; ++k)
+        if (tab.spw[k] <= 0 || tab.R[k] < 1 || !dslash_supported(a.Nc, tab.R[k])) return hipErrorInvalidValue;
+    switch (a.prec) {
+        case kDouble: return dispatch_nc_groups<double, double>(a, tab, by, bz, stream);
+        case kSingle: return dispatch_nc_groups<float, float>(a, tab, by, bz, stream);
+        case kHalf: return dispatch_nc_groups<float, __half>(a, tab, by, bz, stream);
+        case kBF16: return dispatch_nc_groups<float, __hip_bfloat16>(a, tab, by, bz, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -13,6 +13,7 @@ namespace qcu {
 struct GeomTables {
     const int *order = nullptr;  // [Vh]: traversal index -> checkerboard site
     const int *nbr = nullptr;    // [2][Vh][8]: output parity, site, hop 2*mu + (0 fwd | 1 bwd)
+    const int *walk = nullptr;   // [2][Vh][9]: output parity, traversal index -> (site, its 8 neighbours)
 };
 
 // Tables for lattice g and traversal tile (by, bz); built on `stream` on first use.

@@ -27,11 +27,26 @@ __global__ void k_build_nbr(int *nbr, Geom g) {
     }
 }
 
+// one row per traversal position: the site and its 8 neighbours, so a
+// wavefront gets everything it needs for a site with one scalar load
+__global__ void k_build_walk(int *walk, const int *order, const int *nbr, int Vh) {
+    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= 2L * Vh) return;
+    const int parity = int(i / Vh);
+    const int q = int(i - long(parity) * Vh);
+    const int site = order[q];
+    int *row = walk + i * 9;
+    row[0] = site;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) row[1 + k] = nbr[(long(parity) * Vh + site) * 8 + k];
+}
+
 using Key4 = std::tuple<int, int, int, int>;
 using Key6 = std::tuple<int, int, int, int, int, int>;
 std::mutex g_mu;
 std::map<Key4, int *> g_nbr;
 std::map<Key6, int *> g_order;
+std::map<Key6, int *> g_walk;
 
 }  // namespace
 
@@ -58,8 +73,19 @@ hipError_t geom_tables(const Geom &g, int by, int bz, GeomTables &out, hipStream
         QCU_HIP_TRY(hipGetLastError());
         jt = g_order.emplace(k6, p).first;
     }
+    auto wt = g_walk.find(k6);
+    if (wt == g_walk.end()) {
+        built = true;
+        int *p = nullptr;
+        QCU_HIP_TRY(hipMalloc(&p, sizeof(int) * 18 * size_t(g.Vh)));
+        hipLaunchKernelGGL(k_build_walk, dim3(div_up(2L * g.Vh, 256)), dim3(256), 0, stream, p, jt->second,
+                           it->second, g.Vh);
+        QCU_HIP_TRY(hipGetLastError());
+        wt = g_walk.emplace(k6, p).first;
+    }
     out.nbr = it->second;
     out.order = jt->second;
+    out.walk = wt->second;
     // the tables may be read from another stream later: finish building them now
     if (built) QCU_HIP_TRY(hipStreamSynchronize(stream));
     return hipSuccess;
@@ -69,8 +95,10 @@ void geom_tables_release() {
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto &kv : g_nbr) (void)hipFree(kv.second);
     for (auto &kv : g_order) (void)hipFree(kv.second);
+    for (auto &kv : g_walk) (void)hipFree(kv.second);
     g_nbr.clear();
     g_order.clear();
+    g_walk.clear();
 }
 
 }  // namespace qcu

@@ -1,0 +1,312 @@
+// groups.hip -- batched solve of uneven right-hand-side groups (groups.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "dslash.h"
+#include "groups.h"
+
+namespace qcu {
+namespace {
+
+constexpr double kInnerTol = 1e-5;   // relative residual reduction of one inner (storage precision) solve
+constexpr int kMaxOuter = 12;
+constexpr int kGraphIters = 4;       // BiCGStab iterations per graph launch
+
+void *gmalloc(size_t bytes, hipError_t &err) {
+    void *p = nullptr;
+    if (err != hipSuccess) return nullptr;
+    err = hipMalloc(&p, bytes);
+    if (err != hipSuccess) set_error(5, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(err));
+    return err == hipSuccess ? p : nullptr;
+}
+
+}  // namespace
+
+GroupSolver::GroupSolver(const Geom &g, int Nc, const std::vector<int> &Rs)
+    : g_(g), Nc_(Nc), Rs_(Rs), n_(long(g.Vh) * 4 * Nc) {
+    tab_.G = int(Rs.size());
+    tab_.n = n_;
+    int base = 0;
+    for (int k = 0; k < tab_.G; ++k) {
+        tab_.R[k] = Rs[k];
+        tab_.base[k] = base;
+        tab_.spw[k] = 4;
+        base += Rs[k];
+        tab_.Rmax = std::max(tab_.Rmax, Rs[k]);
+    }
+    tab_.Rtot = base;
+}
+
+GroupSolver::~GroupSolver() {
+    for (void *p : {b_, x_, bp_, r64_, t64_, u64_, r_, r0_, p_, v_, s_, t_, tmp_, xl_, static_cast<void *>(st_),
+                    static_cast<void *>(ones_), static_cast<void *>(dn_), static_cast<void *>(scale_),
+                    static_cast<void *>(done_), static_cast<void *>(odone_), static_cast<void *>(iters_)})
+        if (p) (void)hipFree(p);
+    if (h_) (void)hipHostFree(h_);
+    seg_work_free(work_);
+}
+
+SegTab GroupSolver::tab_with(const int *done) const {
+    SegTab t = tab_;
+    t.done = done;
+    return t;
+}
+
+hipError_t GroupSolver::ensure(int prec) {
+    hipError_t err = hipSuccess;
+    const size_t nR = size_t(n_) * tab_.Rtot;
+    if (!b_) {
+        b_ = gmalloc(2 * nR * sizeof(C64), err);
+        x_ = gmalloc(2 * nR * sizeof(C64), err);
+        bp_ = gmalloc(nR * sizeof(C64), err);
+        r64_ = gmalloc(nR * sizeof(C64), err);
+        t64_ = gmalloc(nR * sizeof(C64), err);
+        u64_ = gmalloc(nR * sizeof(C64), err);
+        st_ = static_cast<C64 *>(gmalloc(sizeof(C64) * kNumSlots * tab_.Rtot, err));
+        ones_ = static_cast<C64 *>(gmalloc(sizeof(C64) * tab_.Rtot, err));
+        dn_ = static_cast<double *>(gmalloc(sizeof(double) * 4 * tab_.Rtot, err));
+        scale_ = static_cast<double *>(gmalloc(sizeof(double) * 2 * tab_.Rtot, err));
+        done_ = static_cast<int *>(gmalloc(sizeof(int) * kMaxGroups, err));
+        odone_ = static_cast<int *>(gmalloc(sizeof(int) * kMaxGroups, err));
+        iters_ = static_cast<int *>(gmalloc(sizeof(int) * kMaxGroups, err));
+        if (err != hipSuccess) return err;
+        QCU_HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&h_), sizeof(double) * (8 * tab_.Rtot + 64)));
+        QCU_HIP_TRY(seg_work_alloc(work_, tab_.G, tab_.Rmax));
+        QCU_HIP_TRY(launch_scalar_fill(ones_, C64{1.0, 0.0}, tab_.Rtot, nullptr));
+        QCU_HIP_TRY(hipDeviceSynchronize());
+    }
+    if (prec != lo_prec_) {
+        for (void **p : {&r_, &r0_, &p_, &v_, &s_, &t_, &tmp_, &xl_}) {
+            if (*p) (void)hipFree(*p);
+            *p = gmalloc(nR * 2 * prec_bytes(prec), err);
+        }
+        if (err != hipSuccess) return err;
+        lo_prec_ = prec;
+        // default traversal tile for the wave kernel at this site size
+        DslashArgs a{nullptr, nullptr, nullptr, nullptr, 1.0, g_, Nc_, 16, 1, 0, prec};
+        a.tune.variant = kDslashWaveSite;
+        const DslashTune t = dslash_resolve(a);
+        by_ = t.by;
+        bz_ = t.bz;
+    }
+    return hipSuccess;
+}
+
+hipError_t GroupSolver::dslash(void *out, const void *in, const void *gauge, int prec, int parity,
+                               const void *xpay, double coef, const SegTab &tab, hipStream_t s) {
+    DslashArgs a{out, in, gauge, xpay, coef, g_, Nc_, 0, parity, 0, prec};
+    return launch_dslash_wave_groups(a, tab, by_, bz_, s);
+}
+
+hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
+    QCU_HIP_TRY(ensure(prec));
+    if (tuned_ && tuned_prec_ == prec) return hipSuccess;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QCU_HIP_TRY(hipEventCreate(&e0));
+    QCU_HIP_TRY(hipEventCreate(&e1));
+    // bounded (zero) contents for the timing launches
+    QCU_HIP_TRY(launch_scalar_fill(st_, C64{0.0, 0.0}, kNumSlots * tab_.Rtot, s));
+    QCU_HIP_TRY(hipMemsetAsync(p_, 0, size_t(n_) * tab_.Rtot * 2 * prec_bytes(prec), s));
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < tab_.G && err == hipSuccess; ++k) {
+        // only group k active
+        for (int j = 0; j < kMaxGroups; ++j) reinterpret_cast<int *>(h_)[j] = (j == k) ? 0 : 1;
+        err = hipMemcpyAsync(done_, h_, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        double best = 1e30;
+        int best_spw = tab_.spw[k];
+        for (int spw : {1, 2, 4, 8, 16}) {
+            SegTab t = tab_with(done_);
+            t.spw[k] = spw;
+            if (err == hipSuccess) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);  // warm-up
+            if (err == hipSuccess) err = hipEventRecord(e0, s);
+            for (int rep = 0; rep < 5 && err == hipSuccess; ++rep) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err == hipSuccess && ms < best) {
+                best = ms;
+                best_spw = spw;
+            }
+        }
+        tab_.spw[k] = best_spw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err == hipSuccess) {
+        tuned_ = true;
+        tuned_prec_ = prec;
+    }
+    return err;
+}
+
+// one BiCGStab iteration of every active group (graph-captured)
+hipError_t GroupSolver::iteration(const void *glo, int P, double kappa, hipStream_t s) {
+    const SegTab t = tab_with(done_);
+    const double k2 = -kappa * kappa;
+    QCU_HIP_TRY(dslash(tmp_, p_, glo, P, 0, nullptr, 1.0, t, s));
+    QCU_HIP_TRY(dslash(v_, tmp_, glo, P, 1, p_, k2, t, s));
+    QCU_HIP_TRY(seg_step_alpha(st_, r0_, v_, P, t, work_, s));
+    QCU_HIP_TRY(seg_step_s(s_, r_, v_, st_, P, t, s));
+    QCU_HIP_TRY(dslash(tmp_, s_, glo, P, 0, nullptr, 1.0, t, s));
+    QCU_HIP_TRY(dslash(t_, tmp_, glo, P, 1, s_, k2, t, s));
+    QCU_HIP_TRY(seg_step_omega(st_, t_, s_, P, t, work_, s));
+    QCU_HIP_TRY(seg_step_xr(xl_, r_, p_, s_, t_, r0_, st_, done_, iters_, P, t, work_, s));
+    QCU_HIP_TRY(seg_step_p(p_, r_, v_, st_, P, t, s));
+    return hipSuccess;
+}
+
+hipError_t GroupSolver::solve(const void *g64, const void *glo, int inner_prec, double kappa, int max_iter,
+                              double tol, std::vector<GroupResult> &res, hipStream_t s) {
+    const int P = inner_prec;
+    QCU_HIP_TRY(ensure(P));
+    QCU_HIP_TRY(autotune(glo, P, s));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int G = tab_.G, Rt = tab_.Rtot;
+    const size_t nR = size_t(n_) * Rt;
+    const SegTab all = tab_with(nullptr);
+    const SegTab outer_t = tab_with(odone_);
+    res.assign(G, GroupResult{});
+    for (int k = 0; k < G; ++k) res[k].spw = tab_.spw[k];
+    auto *b_e = static_cast<C64 *>(b_);
+    auto *b_o = b_e + nR;
+    auto *x_e = static_cast<C64 *>(x_);
+    auto *x_o = x_e + nR;
+    double *h = h_;
+    int *hi = reinterpret_cast<int *>(h_ + 6 * Rt);  // [kMaxGroups] flags, then [kMaxGroups] counters
+
+    // b' = b_o + kappa D_oe b_e ; ||b'||^2
+    QCU_HIP_TRY(dslash(bp_, b_e, g64, kDouble, 1, b_o, kappa, all, s));
+    QCU_HIP_TRY(seg_norm(dn_, bp_, kDouble, false, all, work_, 3, s));
+    QCU_HIP_TRY(hipMemcpyAsync(h, dn_, sizeof(double) * Rt, hipMemcpyDeviceToHost, s));
+    QCU_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<double> nb2(h, h + Rt);
+    QCU_HIP_TRY(hipMemsetAsync(x_o, 0, nR * sizeof(C64), s));
+    QCU_HIP_TRY(hipMemcpyAsync(r64_, bp_, nR * sizeof(C64), hipMemcpyDeviceToDevice, s));
+    std::vector<int> odone(kMaxGroups, 1);
+    for (int k = 0; k < G; ++k) odone[k] = 0;
+
+    // inner iteration graph: kGraphIters iterations of every active group
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    QCU_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t cap = hipSuccess;
+    for (int k = 0; k < kGraphIters && cap == hipSuccess; ++k) cap = iteration(glo, P, kappa, s);
+    hipError_t end = hipStreamEndCapture(s, &graph);
+    if (cap != hipSuccess || end != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return cap != hipSuccess ? cap : end;
+    }
+    hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) return err;
+
+    auto finish = [&](hipError_t e) {
+        (void)hipGraphExecDestroy(exec);
+        return e;
+    };
+    for (int outer = 0; outer < kMaxOuter; ++outer) {
+        // FP64 residual norms; groups whose every RHS meets the tolerance stop
+        err = seg_norm(dn_, r64_, kDouble, false, all, work_, 4, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(h, dn_, sizeof(double) * Rt, hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        if (err != hipSuccess) return finish(err);
+        bool all_done = true;
+        const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int k = 0; k < G; ++k) {
+            if (odone[k]) continue;
+            bool ok = true;
+            for (int r = 0; r < tab_.R[k]; ++r) {
+                const int q = tab_.base[k] + r;
+                if (!(h[q] <= tol * tol * nb2[q])) ok = false;
+            }
+            if (ok) {
+                odone[k] = 1;
+                res[k].converged = 1;
+                res[k].seconds = now;
+            } else if (res[k].iterations >= max_iter) {
+                odone[k] = 1;  // iteration budget spent: stop this group unconverged
+                res[k].seconds = now;
+            }
+            all_done = all_done && odone[k];
+        }
+        if (all_done) break;
+        // scale each RHS residual to unit norm for the inner solve
+        for (int q = 0; q < Rt; ++q) {
+            const double nr = std::sqrt(h[q]);
+            h[Rt + q] = nr > 0 ? 1.0 / nr : 0.0;  // to inner
+            h[2 * Rt + q] = nr;                   // back to FP64
+        }
+        for (int j = 0; j < kMaxGroups; ++j) {
+            hi[j] = odone[j];
+            hi[kMaxGroups + j] = 0;
+        }
+        err = hipMemcpyAsync(scale_, h + Rt, sizeof(double) * 2 * Rt, hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(odone_, hi, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(done_, hi, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(iters_, hi + kMaxGroups, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+        // inner start: r = r0 = p = r64 / ||r64||, x = 0, scalars, threshold, rho
+        if (err == hipSuccess) err = seg_convert(r_, P, r64_, kDouble, scale_, outer_t, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(r0_, r_, nR * 2 * prec_bytes(P), hipMemcpyDeviceToDevice, s);
+        if (err == hipSuccess) err = hipMemcpyAsync(p_, r_, nR * 2 * prec_bytes(P), hipMemcpyDeviceToDevice, s);
+        if (err == hipSuccess) err = hipMemsetAsync(xl_, 0, nR * 2 * prec_bytes(P), s);
+        if (err == hipSuccess) err = launch_scalar_fill(st_, C64{0.0, 0.0}, kNumSlots * Rt, s);
+        if (err == hipSuccess) err = launch_scalar_fill(st_ + kBnorm2 * Rt, C64{kInnerTol * kInnerTol, 0.0}, Rt, s);
+        if (err == hipSuccess) err = seg_dotc(st_ + kRho * Rt, r0_, r_, P, outer_t, work_, 6, s);
+        if (err == hipSuccess) err = hipStreamSynchronize(s);
+        if (err != hipSuccess) return finish(err);
+        // replay until every active group's inner solve is done or out of iterations
+        const int budget = max_iter;
+        for (int launched = 0; launched < budget; launched += kGraphIters) {
+            err = hipGraphLaunch(exec, s);
+            if (err == hipSuccess) err = hipMemcpyAsync(hi, done_, sizeof(int) * kMaxGroups, hipMemcpyDeviceToHost, s);
+            if (err == hipSuccess) err = hipMemcpyAsync(hi + kMaxGroups, iters_, sizeof(int) * kMaxGroups, hipMemcpyDeviceToHost, s);
+            if (err == hipSuccess) err = hipStreamSynchronize(s);
+            if (err != hipSuccess) return finish(err);
+            bool inner_done = true;
+            for (int k = 0; k < G; ++k) inner_done = inner_done && (hi[k] || res[k].iterations + hi[kMaxGroups + k] >= max_iter);
+            if (inner_done) break;
+        }
+        for (int k = 0; k < G; ++k)
+            if (!odone[k]) {
+                res[k].iterations += hi[kMaxGroups + k];
+                res[k].outer_steps += 1;
+            }
+        // x += ||r|| d (FP64) and r = b' - A x (FP64) for the groups still running
+        err = seg_accum(x_o, xl_, P, scale_ + Rt, outer_t, s);
+        if (err == hipSuccess) err = dslash(t64_, x_o, g64, kDouble, 0, nullptr, 1.0, outer_t, s);
+        if (err == hipSuccess) err = dslash(u64_, t64_, g64, kDouble, 1, x_o, -kappa * kappa, outer_t, s);
+        if (err == hipSuccess) err = seg_xsay(r64_, bp_, u64_, ones_, kDouble, outer_t, s);
+        if (err != hipSuccess) return finish(err);
+    }
+    (void)finish(hipSuccess);
+    // x_e = b_e + kappa D_eo x_o ; FP64 true residual of the full system per RHS
+    QCU_HIP_TRY(dslash(x_e, x_o, g64, kDouble, 0, b_e, kappa, all, s));
+    std::vector<double> rr(Rt, 0.0), bb(Rt, 0.0);
+    for (int p = 0; p < 2; ++p) {
+        const C64 *xp = p ? x_o : x_e, *xq = p ? x_e : x_o, *bpar = p ? b_o : b_e;
+        QCU_HIP_TRY(dslash(t64_, xq, g64, kDouble, p, xp, -kappa, all, s));
+        QCU_HIP_TRY(seg_xsay(u64_, bpar, t64_, ones_, kDouble, all, s));
+        QCU_HIP_TRY(seg_norm(dn_, u64_, kDouble, false, all, work_, 4, s));
+        QCU_HIP_TRY(seg_norm(dn_ + Rt, bpar, kDouble, false, all, work_, 5, s));
+        QCU_HIP_TRY(hipMemcpyAsync(h, dn_, sizeof(double) * 2 * Rt, hipMemcpyDeviceToHost, s));
+        QCU_HIP_TRY(hipStreamSynchronize(s));
+        for (int q = 0; q < Rt; ++q) {
+            rr[q] += h[q];
+            bb[q] += h[Rt + q];
+        }
+    }
+    for (int k = 0; k < G; ++k) {
+        double mx = 0.0;
+        for (int r = 0; r < tab_.R[k]; ++r) {
+            const int q = tab_.base[k] + r;
+            mx = std::max(mx, bb[q] > 0 ? std::sqrt(rr[q] / bb[q]) : std::sqrt(rr[q]));
+        }
+        res[k].true_rel_residual = mx;
+    }
+    return hipSuccess;
+}
+
+}  // namespace qcu

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "groups.h"
 #include "layout.h"
 #include "lgnn.h"
 #include "qcu.h"
@@ -59,6 +60,7 @@ struct Context {
     void *bufA = nullptr, *bufB = nullptr;  // full-lattice MRHS vectors, FP64-sized
     std::vector<void *> q_in, q_out;
     std::unique_ptr<BiCGStab> solver;
+    std::unique_ptr<GroupSolver> groups;  // batched uneven RHS groups (qcuInvertGroups)
     SolverOptions opts;
     std::unique_ptr<LgnnPreconditioner> precond;
     struct LgnnBatch {
@@ -84,6 +86,7 @@ struct Context {
 
     ~Context() {
         clear_lgnn_data();
+        groups.reset();
         solver.reset();
         precond.reset();
         for (void *p : {gauge64, bufA, bufB}) if (p) (void)hipFree(p);
@@ -578,6 +581,88 @@ int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precisio
     t.bz = bz;
     t.spw = spw;
     dslash_set_tuned(lattice, n_color, m_rhs, precision, t);
+    return 0;
+}
+
+// ---- batched uneven RHS groups ------------------------------------------
+int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, double tol, QcuGroupInfo *info) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->have_dslash) {
+        set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
+        return g_err;
+    }
+    if (n_groups < 1 || n_groups > kMaxGroups || !groups || max_iteration < 1 || !(tol > 0.0)) {
+        set_error(kQcuErrorInvalidValue, "qcuInvertGroups: 1..%d groups, max_iteration >= 1, tol > 0", kMaxGroups);
+        return g_err;
+    }
+    std::vector<int> Rs;
+    for (int k = 0; k < n_groups; ++k) {
+        const QcuRhsGroup &gr = groups[k];
+        if (gr.m_rhs < 1 || gr.m_rhs > kMaxPtrs || !gr.fermion_in || !gr.fermion_out || !dslash_supported(c->Nc, gr.m_rhs)) {
+            set_error(kQcuErrorInvalidValue, "qcuInvertGroups: group %d has an invalid size or pointer table", k);
+            return g_err;
+        }
+        for (int r = 0; r < gr.m_rhs; ++r)
+            if (!gr.fermion_in[r] || !gr.fermion_out[r]) {
+                set_error(kQcuErrorInvalidValue, "qcuInvertGroups: null fermion pointer in group %d", k);
+                return g_err;
+            }
+        Rs.push_back(gr.m_rhs);
+    }
+    if (!c->groups || c->groups->sizes() != Rs) {
+        c->groups.reset();
+        c->groups = std::make_unique<GroupSolver>(c->g, c->Nc, Rs);
+    }
+    GroupSolver &gs = *c->groups;
+    const int P = c->opts.inner_prec;
+    const void *glo = c->gauge(P);
+    if (!glo) return g_err;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<GroupResult> res;
+    // batch buffers and the per-group launch shapes (timed once per batch shape and precision)
+    hipError_t e = gs.autotune(glo, P, c->stream);
+    const long n = c->half_n();
+    const long nR = n * gs.total_rhs();
+    auto *b = static_cast<Cx<double> *>(gs.b_full());
+    int base = 0;
+    for (int k = 0; k < n_groups && e == hipSuccess; ++k) {
+        std::vector<void *> in(groups[k].fermion_in, groups[k].fermion_in + Rs[k]);
+        e = launch_gather(b + n * base, kDouble, table(in, 0, c->user_prec), c->user_prec, n, Rs[k], c->stream);
+        if (e == hipSuccess)
+            e = launch_gather(b + nR + n * base, kDouble, table(in, n, c->user_prec), c->user_prec, n, Rs[k], c->stream);
+        base += Rs[k];
+    }
+    if (e == hipSuccess) e = gs.solve(c->gauge64, glo, P, c->kappa, max_iteration, tol, res, c->stream);
+    auto *x = static_cast<Cx<double> *>(gs.x_full());
+    base = 0;
+    for (int k = 0; k < n_groups && e == hipSuccess; ++k) {
+        std::vector<void *> out(groups[k].fermion_out, groups[k].fermion_out + Rs[k]);
+        e = launch_scatter(table(out, 0, c->user_prec), c->user_prec, x + n * base, kDouble, n, Rs[k], c->stream);
+        if (e == hipSuccess)
+            e = launch_scatter(table(out, n, c->user_prec), c->user_prec, x + nR + n * base, kDouble, n, Rs[k], c->stream);
+        base += Rs[k];
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (!hip_ok(e, "qcuInvertGroups")) return g_err;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    bool all = true;
+    for (int k = 0; k < n_groups; ++k) {
+        all = all && res[k].converged;
+        if (info) {
+            info[k].converged = res[k].converged;
+            info[k].iterations = res[k].iterations;
+            info[k].outer_steps = res[k].outer_steps;
+            info[k].sites_per_wave = res[k].spw;
+            info[k].true_rel_residual = res[k].true_rel_residual;
+            info[k].seconds = res[k].seconds;
+            info[k].batch_seconds = secs;
+        }
+    }
+    if (!all) {
+        set_error(kQcuErrorNotConverged, "qcuInvertGroups: not every group converged in %d iterations", max_iteration);
+        return kQcuErrorNotConverged;
+    }
     return 0;
 }
 

@@ -49,6 +49,23 @@ struct Geom {
     int Vh;      // half volume
 };
 
+// A batch of right-hand-side groups stored back to back ("segments"): in every
+// batch half vector, group g occupies entries [n*base[g], n*(base[g]+R[g])) in
+// the [n][R[g]] layout (n = Vh*4*Nc entries per RHS); its per-RHS solver
+// scalars are st[slot*Rtot + base[g] + r].  Batched kernels put the group in
+// blockIdx.y and skip a group whose done flag is set.
+constexpr int kMaxGroups = 8;
+struct SegTab {
+    int G = 0;
+    int Rtot = 0;                  // RHS over all groups (scalar slot stride)
+    int Rmax = 0;
+    long n = 0;                    // complex entries per RHS of a half vector
+    int R[kMaxGroups] = {};
+    int base[kMaxGroups] = {};     // first batch-wide RHS index of the group
+    int spw[kMaxGroups] = {};      // dslash sites per wavefront for the group (autotuned)
+    const int *done = nullptr;     // device flags [G]; null = every group active
+};
+
 inline Geom make_geom(const int L[4]) {
     Geom g;
     for (int i = 0; i < 4; ++i) g.L[i] = L[i];
