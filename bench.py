@@ -98,6 +98,47 @@ def measured_traffic(workload, prec_name, variant):
     return None
 
 
+def train_network(w, args):
+    """Learned preconditioner for the workload's kappa and link ensemble,
+    trained on the GPU with the library's own kernels: 16^3 x 32 lattices
+    (the weights do not depend on the volume), `train_configs` seeded
+    configurations, 16 sources each, 16 iterate snapshots + 16 Gaussian errors
+    per source (4096 pairs for 8 configurations), Adam with cosine decay.
+    Returns (W1, W2, per-epoch losses, seconds)."""
+    from qcu_mrhs_amd import lgnn
+
+    Ltr, Nc, R = (16, 16, 16, 32), w["Nc"], 16
+    t0 = time.perf_counter()
+    api.initGridSize(Ltr, Nc, R, 2, 2)
+    d_g = None
+    w1, w2 = lgnn.neumann_weights(w["kappa"], channels=8, noise=1e-3, seed=11)
+    for k in range(args.train_configs):
+        gauge = fields.gauge_field(Ltr, Nc, 7000 + k, kind=w["kind"], eps=w["eps"])
+        if d_g is None:
+            d_g = api.DeviceArray.from_complex(gauge.reshape(-1))
+        else:
+            d_g.upload(gauge.reshape(-1))
+        api.loadQcuGauge(d_g, 2)
+        if k == 0:
+            api.getDslash(0, 1.0 / (2.0 * w["kappa"]) - 4.0)
+            lgnn.set_weights(w1, w2)
+            lgnn.clear_data()
+        b = fields.gaussian_sources(Ltr, Nc, R, 7100 + k)
+        bs = [api.DeviceArray.from_complex(b[r]) for r in range(R)]
+        xs = [api.DeviceArray(b[r].nbytes) for r in range(R)]
+        for r in range(R):
+            api.pushBackFermions(xs[r], bs[r])
+        lgnn.collect(args.max_iter, 1e-10, 16, n_random=16, seed=k)
+    losses = lgnn.train(args.train_epochs, 2e-3, seed=5)
+    tw = lgnn.get_weights(8)
+    lgnn.clear_data()
+    api.finalizeQcu()
+    secs = time.perf_counter() - t0
+    log(f"learned preconditioner: {args.train_configs} configurations, loss {losses[0]:.4f} -> "
+        f"{losses[-1]:.4f} in {args.train_epochs} epochs, {secs:.1f} s")
+    return tw[0], tw[1], [float(v) for v in losses], secs
+
+
 def solve_once(d_b, d_x, R, max_iter, tol):
     for r in range(R):
         api.pushBackFermions(d_x[r], d_b[r])
@@ -134,6 +175,51 @@ def cpu_baseline(w, gauge, b, plain_iters):
     }
 
 
+def bench_groups(args):
+    """Batched uneven groups: 16^4 lattice, Haar SU(N) links, kappa 0.12,
+    sources in groups of 3, 8, 17 and 32, one qcuInvertGroups call per step.
+    Reports sources solved per second and each group's solve time."""
+    L, kappa, sizes = (16, 16, 16, 16), 0.12, (3, 8, 17, 32)
+    out = {"metric": "batched uneven-group solve throughput (sources to 1e-10 per second)", "unit": "rhs/s",
+           "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "data": "synthetic (Haar SU(N) links, Gaussian sources)", "dtype": "f32 storage inner, f64 outer",
+           "config": {"workload": "SU(N) Wilson 16^4, groups of 3, 8, 17, 32 sources, kappa 0.12, tol 1e-10",
+                      "lattice": list(L), "groups": list(sizes)}, "results": {}}
+    for Nc in (4, 8):
+        gauge = fields.gauge_field(L, Nc, 5000 + Nc, kind="haar")
+        api.initGridSize(L, Nc, 4, 2, 1)
+        d_g = api.DeviceArray.from_complex(gauge.reshape(-1))
+        api.loadQcuGauge(d_g, 2)
+        api.getDslash(0, 1.0 / (2 * kappa) - 4.0)
+        api.set_solver_options(1, args.delta, False)
+        groups = []
+        for k, R in enumerate(sizes):
+            b = fields.gaussian_sources(L, Nc, R, 5100 + 10 * Nc + k)
+            groups.append(([api.DeviceArray(b[r].nbytes) for r in range(R)],
+                           [api.DeviceArray.from_complex(b[r]) for r in range(R)]))
+        for _ in range(args.warmup):
+            api.invert_groups(groups, args.max_iter, args.tol)
+        runs = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            info = api.invert_groups(groups, args.max_iter, args.tol)
+            runs.append((time.perf_counter() - t0, info))
+        wall = float(np.mean([r[0] for r in runs]))
+        info = runs[-1][1]
+        res = {"batch_s": wall, "rhs_per_s": sum(sizes) / wall,
+               "groups": [{"m_rhs": R, "converged": bool(i["converged"]), "iterations": i["iterations"],
+                           "outer_steps": i["outer_steps"], "sites_per_wave": i["sites_per_wave"],
+                           "solve_s": i["seconds"], "true_rel_residual": i["true_rel_residual"]}
+                          for R, i in zip(sizes, info)]}
+        log(f"SU({Nc}): batch {wall * 1e3:.1f} ms, {res['rhs_per_s']:.1f} rhs/s; " + ", ".join(
+            f"R={g['m_rhs']}: {g['iterations']} it, {g['solve_s'] * 1e3:.1f} ms, {g['true_rel_residual']:.1e}"
+            for g in res["groups"]))
+        out["results"][f"su{Nc}"] = res
+        api.finalizeQcu()
+    out["value"] = out["results"]["su4"]["rhs_per_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +233,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--learned", default="half",
+                    help="inner precisions also timed with the learned preconditioner (comma list; "
+                         "'' = none); the network is trained first on 16^3x32 configurations")
+    ap.add_argument("--train-configs", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=12)
     ap.add_argument("--dslash-only", action="store_true",
                     help="time only the dslash (all storage precisions) and a plain copy kernel "
                          "of one half vector (counter calibration)")
@@ -154,11 +245,18 @@ def main():
                     help="dslash kernel variant for every launch (default: automatic choice)")
     ap.add_argument("--dslash-precs", default="double,single,half,bf16",
                     help="storage precisions timed by the dslash measurement")
+    ap.add_argument("--groups", action="store_true",
+                    help="batched uneven RHS groups instead: SU(4) and SU(8) on 16^4, groups of 3, 8, 17 "
+                         "and 32 sources, FP32 inner, tol 1e-10 (prints its own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (multi-GPU sharding is "
               "design-only for this solver)", file=sys.stderr)
         sys.exit(2)
+
+    if args.groups:
+        bench_groups(args)
+        return
 
     w = WORKLOADS[args.workload]
     L, Nc, R, kappa = w["L"], w["Nc"], w["R"], w["kappa"]
@@ -206,6 +304,9 @@ def main():
     d_src.free()
     del half_bytes
 
+    learned_methods = [m for m in args.learned.split(",") if m]
+    trained = train_network(w, args) if learned_methods else None
+
     api.initGridSize(L, Nc, R, 2, 2)
     api.loadQcuGauge(d_gauge, 2)
     api.getDslash(0, 1.0 / (2.0 * kappa) - 4.0)
@@ -221,12 +322,18 @@ def main():
         results["plain_fp64"] = info
 
     timed = {}
-    for m in [s for s in args.methods.split(",") if s]:
-        api.set_solver_options(PREC[m], args.delta, False)
+    plan = [(m, False) for m in args.methods.split(",") if m] + [(m, True) for m in learned_methods]
+    if trained is not None:
+        from qcu_mrhs_amd import lgnn
+        lgnn.set_weights(trained[0], trained[1])
+    for m0, use_pc in plan:
+        m = m0 + ("+learned" if use_pc else "")
+        api.set_solver_options(PREC[m0], args.delta, use_pc)
         for _ in range(args.warmup):
             solve_once(d_b, d_x, R, args.max_iter, args.tol)
         runs = [solve_once(d_b, d_x, R, args.max_iter, args.tol) for _ in range(args.steps)]
-        ok = all(r["converged"] and r["true_rel_residual"] <= args.tol for r in runs)
+        ok = all(r["converged"] and r["true_rel_residual"] <= args.tol and not r.get("fallback", 0)
+                 for r in runs)
         mean = float(np.mean([r["wall_s"] for r in runs]))
         timed[m] = {"mean_s": mean, "ok": ok, "runs": runs}
         log(f"mixed {m}: {mean:.3f} s/solve over {args.steps}, {runs[-1]['iterations']} iterations, "
@@ -238,8 +345,9 @@ def main():
     if good:
         head = min(good, key=lambda m: good[m]["mean_s"])
         value = good[head]["mean_s"]
-        head_prec = PREC[head]
-        method = f"reliable-update mixed precision, {head} inner dslash/vectors, FP64 reductions/updates"
+        head_prec = PREC[head.split("+")[0]]
+        method = (f"reliable-update mixed precision, {head.split('+')[0]} inner dslash/vectors, FP64 "
+                  f"reductions/updates" + (", learned right preconditioner" if "+learned" in head else ""))
     else:
         head, value, head_prec = "plain_fp64", plain["wall_s"] if plain else None, 2
         method = "plain FP64 BiCGStab"
@@ -286,6 +394,9 @@ def main():
                         "true_rel_residual": max(r["true_rel_residual"] for r in v["runs"])}
                     for m, v in timed.items()},
         "dslash": dslash,
+        "learned": ({"train_configs": args.train_configs, "train_lattice": [16, 16, 16, 32],
+                     "epochs": args.train_epochs, "loss": trained[2], "train_s": trained[3]}
+                    if trained is not None else None),
     }
     print(json.dumps(out))
 

@@ -85,12 +85,14 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
     if (active) {
         auto loc = op.local(rhs);
         long e = g;
-        // two entries per trip keep more loads in flight (same summation order as one per trip)
-        for (; e + stride < N; e += 2 * stride) {
+        // four entries per trip keep more loads in flight (same summation order as one per trip)
+        for (; e + 3 * stride < N; e += 4 * stride) {
             op.elem(e, loc, acc);
             op.elem(e + stride, loc, acc);
+            op.elem(e + 2 * stride, loc, acc);
+            op.elem(e + 3 * stride, loc, acc);
         }
-        if (e < N) op.elem(e, loc, acc);
+        for (; e < N; e += stride) op.elem(e, loc, acc);
     }
 #pragma unroll
     for (int k = 0; k < NA; ++k) buf[k][threadIdx.x] = acc[k];
@@ -122,7 +124,21 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
     if (t < per_block) {
         const int r = t % R, ch = t / R;
-        for (int bb = ch; bb < int(gridDim.x); bb += nch)
+        // four independent partial loads per trip (latency of the L2 reads overlaps);
+        // fixed order: bb, bb + nch, bb + 2 nch, bb + 3 nch, then the next four
+        int bb = ch;
+        for (; bb + 3 * nch < int(gridDim.x); bb += 4 * nch) {
+            double v[4][NA];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < NA; ++k) v[u][k] = partials[(long(bb + u * nch) * R + r) * 4 + k];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < NA; ++k) acc[k] += v[u][k];
+        }
+        for (; bb < int(gridDim.x); bb += nch)
 #pragma unroll
             for (int k = 0; k < NA; ++k) acc[k] += partials[(long(bb) * R + r) * 4 + k];
     }
@@ -470,7 +486,7 @@ int reduce_blocks(long n, int R, int max_blocks) {
 }
 
 hipError_t reduce_work_alloc(ReduceWork &w, int R) {
-    w.max_blocks = 4096;  // 16 workgroups per CU: enough loads in flight for 4-byte storage
+    w.max_blocks = 1024;
     QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R));
     QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots));
     QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots));

@@ -24,6 +24,8 @@ struct SolverOptions {
     double delta = 0.1;        // reliable update when ||r|| < delta * max ||r|| since last update
     int use_precond = 0;       // apply the learned right preconditioner (inner solve only)
     double diverge = 1e6;      // stop (not converged) once a relative residual estimate exceeds this
+    int verify_residual = 0;   // recompute b - M x on the full lattice after a mixed solve (else the
+                               // final reliable update's FP64 residual, which equals it, is reported)
     int verbose = 0;
 };
 
