@@ -207,7 +207,7 @@ DslashTune dslash_resolve(const DslashArgs &a) {
     if (t.variant == kDslashAuto) t.variant = dslash_mfma_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
     if (t.variant == kDslashMfma && !dslash_mfma_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
     if (t.variant != kDslashLaneRhs) {
-        if (t.spw <= 0) t.spw = t.variant == kDslashMfma ? 8 : 4;  // measured best at 24^3x48, R = 16
+        if (t.spw <= 0) t.spw = t.variant == kDslashMfma ? 1 : 4;  // measured best at 24^3x48, R = 16
         // tile the (y, z) plane so one tile slice (the t-reuse distance) stays
         // around 1 MiB: bytes per site = 4 Nc R complex numbers
         const double site_bytes = 4.0 * a.Nc * a.R * 2.0 * double(prec_bytes(a.prec));

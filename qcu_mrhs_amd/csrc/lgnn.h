@@ -28,6 +28,15 @@ constexpr int kLgnnHops = 9;
 // function of (key, i): SplitMix64 counter hashing + Box-Muller.
 hipError_t launch_gaussian(Cx<double> *out, long count, uint64_t key, hipStream_t s);
 
+// FP16 / BF16 network application on matrix cores (lgnn_mfma.hip), Nc <= 4:
+// q_k(y) = sum_k' M[k][k'] (T_k' in)(y) on the even sites (transport on VALU,
+// the 32 x 32 (hop, spin) mixing on MFMA), then out(x) = M0 in(x) + sum_k
+// (T_k q_k)(x) on the odd sites (transport on MFMA).  q holds 8 fields of one
+// 32-bit word per complex; mcol = the collapsed stencil (M [32][32], M0 [4][4]).
+constexpr size_t kLgnnFragBytes = 8 * 64 * 16;  // MFMA fragments of M stored after the q fields
+hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
+                            const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s);
+
 class LgnnPreconditioner : public Preconditioner {
 public:
     LgnnPreconditioner(const Geom &g, int Nc, int R, int channels);
@@ -60,6 +69,8 @@ public:
 
 private:
     hipError_t ensure_fields(hipStream_t s);
+    hipError_t ensure_work32(hipStream_t s);
+    hipError_t apply_mfma(void *out, const void *in, int prec, hipStream_t s);
     hipError_t collapse(hipStream_t s);
     hipError_t forward(const void *in, int prec, void *out, int out_prec, hipStream_t s);
 
@@ -82,6 +93,10 @@ private:
     int gram_blocks_ = 0;
     double *dnorm_ = nullptr, *dloss_ = nullptr, *hbuf_ = nullptr;
     ReduceWork red_;
+    // FP16 / BF16 inference path: gauge copies [half, bf16] and the 8 mixed hop fields
+    void *glo_[2] = {nullptr, nullptr};
+    bool glo_valid_[2] = {false, false};
+    void *q16_ = nullptr;
 };
 
 }  // namespace qcu

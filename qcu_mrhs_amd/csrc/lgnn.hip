@@ -388,7 +388,7 @@ LgnnPreconditioner::LgnnPreconditioner(const Geom &g, int Nc, int R, int channel
 LgnnPreconditioner::~LgnnPreconditioner() {
     for (void *p : {(void *)w1_, (void *)w2_, (void *)g1_, (void *)g2_, (void *)m1_, (void *)m2_, (void *)v1_,
                     (void *)v2_, (void *)mcol_, u_, q_, inf_, phi_, dlt_, (void *)gm_, (void *)dnorm_,
-                    (void *)dloss_, gauge32_})
+                    (void *)dloss_, gauge32_, q16_, glo_[0], glo_[1]})
         if (p) (void)hipFree(p);
     if (hbuf_) (void)hipHostFree(hbuf_);
     reduce_work_free(red_);
@@ -398,11 +398,10 @@ bool LgnnPreconditioner::supports(int prec) const {
     return prec == kDouble || prec == kSingle || prec == kHalf || prec == kBF16;
 }
 
-hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
+hipError_t LgnnPreconditioner::ensure_work32(hipStream_t) {
     if (u_) return hipSuccess;
     hipError_t err = hipSuccess;
     const size_t half32 = size_t(n_) * R_ * sizeof(C32);
-    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
     u_ = dmal(8 * half32, err);
     q_ = dmal(8 * half32, err);
     inf_ = dmal(half32, err);
@@ -410,6 +409,14 @@ hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
     dlt_ = dmal(half32, err);
     gram_blocks_ = 512;
     gm_ = static_cast<C64 *>(dmal(sizeof(C64) * 1040 * size_t(gram_blocks_), err));
+    if (err != hipSuccess) set_error(5, "learned preconditioner: allocation failed (%s)", hipGetErrorString(err));
+    return err;
+}
+
+hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
+    if (w1_) return hipSuccess;
+    hipError_t err = hipSuccess;
+    const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
     mcol_ = static_cast<Cx<float> *>(dmal(sizeof(Cx<float>) * 1040, err));
     w1_ = static_cast<C64 *>(dmal(wbytes, err));
     w2_ = static_cast<C64 *>(dmal(wbytes, err));
@@ -433,6 +440,7 @@ hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
 
 hipError_t LgnnPreconditioner::set_gauge(const void *gauge64, hipStream_t s) {
     gauge_ = gauge64;
+    glo_valid_[0] = glo_valid_[1] = false;
     const long count = 8L * g_.Vh * Nc_ * Nc_;
     if (!gauge32_) QCU_HIP_TRY(hipMalloc(&gauge32_, sizeof(C32) * count));
     return launch_convert(gauge32_, kSingle, gauge64, kDouble, count, nullptr, 1, s);
@@ -464,6 +472,7 @@ hipError_t LgnnPreconditioner::collapse(hipStream_t s) {
 }
 
 hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int out_prec, hipStream_t s) {
+    QCU_HIP_TRY(ensure_work32(s));
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
     const int spb = R_ >= 256 ? 1 : 256 / R_;
@@ -519,7 +528,24 @@ hipError_t LgnnPreconditioner::apply(void *out, const void *in, int prec, hipStr
         return hipErrorInvalidValue;
     }
     QCU_HIP_TRY(ensure_fields(s));
+    if ((prec == kHalf || prec == kBF16) && Nc_ <= 4) return apply_mfma(out, in, prec, s);
     return forward(in, prec, out, prec, s);
+}
+
+hipError_t LgnnPreconditioner::apply_mfma(void *out, const void *in, int prec, hipStream_t s) {
+    const int slot = prec == kHalf ? 0 : 1;
+    const long gcount = 8L * g_.Vh * Nc_ * Nc_;
+    const size_t vec = size_t(n_) * R_ * 4;  // one 16-bit complex per word
+    if (double(vec) * 8.0 >= 2147483647.0 * 4.0 || double(gcount) * 4.0 >= 2147483647.0) return hipErrorInvalidValue;
+    if (!glo_[slot]) QCU_HIP_TRY(hipMalloc(&glo_[slot], size_t(gcount) * 4));
+    if (!glo_valid_[slot]) {
+        QCU_HIP_TRY(launch_convert(glo_[slot], prec, gauge_, kDouble, gcount, nullptr, 1, s));
+        glo_valid_[slot] = true;
+    }
+    if (!q16_) QCU_HIP_TRY(hipMalloc(&q16_, 8 * vec + kLgnnFragBytes));
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
+    return launch_lgnn_mfma(out, in, glo_[slot], q16_, mcol_, tab.walk, g_, Nc_, R_, prec, s);
 }
 
 hipError_t LgnnPreconditioner::zero_grad(hipStream_t s) {

@@ -1,0 +1,348 @@
+// lgnn_mfma.hip -- FP16 / BF16 application of the learned preconditioner on
+// matrix cores (declared in lgnn.h).
+//
+// Network (collapsed, odd-parity input psi):
+//     out(x) = M0 psi(x) + sum_k (T_k q_k)(x),   q_k(y) = sum_k' M[k][k'] (T_k' psi)(y)
+// with hops k = 2 mu + (0 forward | 1 backward), T_k f(y) = V_k(y) f(nb_k(y)),
+// V = U_mu(y) forward, U_mu(y - mu)^+ backward, M a 32 x 32 complex matrix over
+// (hop, spin) and M0 a 4 x 4 spin matrix.
+//
+// k_lgnn_mix: one wavefront per (even site, 16-RHS chunk).  Lane (m, g)
+// (m = RHS, g = lane group) transports, for one colour at a time, the 4 spins
+// of hop k' = 4 st + g on VALU (FP32) and packs them as the MFMA B operand
+// (k index = (k', spin, re|im)); the real 2x2 embedding of M is the A operand
+// (rows = (k, s, re|im)), kept in registers for the whole wavefront.  8 MFMAs
+// (4 row blocks x 2 k steps) per colour give all 32 q entries of that colour.
+// k_lgnn_gather: one wavefront per (odd site, 16-RHS chunk), the dslash's MFMA
+// structure without spin projection: links as the A operand, the 4 spins of
+// q_k(nb_k) as B, one MFMA chain per spin; M0 psi(x) is added in the epilogue.
+// Both kernels read a walk-table row (site + 8 neighbours) per site.
+#include <cstdint>
+
+#include "lgnn.h"
+
+namespace qcu {
+namespace {
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename S> struct W;
+template <> struct W<__half> {
+    __device__ __forceinline__ static float re(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[0]); }
+    __device__ __forceinline__ static float im(uint32_t w) { return float(__builtin_bit_cast(f16x2, w)[1]); }
+    __device__ __forceinline__ static uint32_t pack(float r, float i) {
+        const f16x2 v = {_Float16(r), _Float16(i)};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+    __device__ __forceinline__ static f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
+                                                      c, 0, 0, 0);
+    }
+};
+template <> struct W<__hip_bfloat16> {
+    __device__ __forceinline__ static float re(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+    __device__ __forceinline__ static float im(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+    __device__ __forceinline__ static uint32_t pack(float r, float i) {
+        const bf16x2 v = {__bf16(r), __bf16(i)};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+    __device__ __forceinline__ static f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+};
+
+constexpr int kRawBufferFlags = 0x00020000;  // gfx9 raw buffer resource, dword 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, int(bytes), kRawBufferFlags);
+}
+__device__ __forceinline__ uint32_t bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), 0);
+}
+
+// MFMA A-operand fragments of the real 2x2 embedding of M, in lane order:
+// frag[(rb * 2 + st) * 64 + lane] = 4 words (8 values) of rows (k, s, re|im) =
+// rb*16 + lane%16 and columns (k', t, re|im) = st*32 + (lane/16)*8 + 0..7.
+template <typename S>
+__global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restrict__ mcol) {
+    using P = W<S>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * 64) return;
+    const int lane = i & 63, rbst = i >> 6, rb = rbst >> 1, st = rbst & 1;
+    const int m = lane & 15, g = lane >> 4;
+    u32x4 a;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = rb * 16 + m, col = st * 32 + g * 8 + 2 * w + h;
+            const int ks = row >> 1, rho = row & 1, kt = col >> 1, sig = col & 1;
+            const Cx<float> c = mcol[ks * 32 + kt];
+            v[h] = rho == sig ? c.re : (rho == 0 ? -c.im : c.im);
+        }
+        a[w] = P::pack(v[0], v[1]);
+    }
+    frag[i] = a;
+}
+
+// q fields: 8 back-to-back half vectors [8][Vh][4][NC][R] of 32-bit words;
+// `fbytes` = bytes of one field (< 2^31, checked on the host).
+template <typename S, int NC>
+__global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, const uint32_t *__restrict__ in,
+                                                 const uint32_t *__restrict__ gauge, const u32x4 *__restrict__ frag,
+                                                 const int *__restrict__ walk0, int Vh, int R, unsigned fbytes,
+                                                 unsigned gauge_bytes) {
+    using P = W<S>;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, g = lane >> 4;
+    const int pos = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    if (pos >= Vh) return;  // uniform
+    const int r = 16 * int(blockIdx.y) + m;
+    const unsigned row_bytes = unsigned(R) * 4u, site_bytes = 4u * NC * row_bytes;
+    const unsigned rcb = unsigned(min(r, R - 1)) * 4u;
+    const auto r_in = rsrc(in, fbytes);
+    const auto r_g = rsrc(gauge, gauge_bytes);
+
+    // A operand: real embedding of M (precomputed fragments, one 16-byte load each)
+    u32x4 A[4][2];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int st = 0; st < 2; ++st) A[rb][st] = frag[(rb * 2 + st) * 64 + lane];
+
+    const int *row = walk0 + long(pos) * 9;
+    const int site = row[0];
+    // this lane's hop in each k step: k' = 4 st + g
+    uint32_t psi[2][4][NC], lk[2][NC][NC];
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        const int h = 4 * st + g, mu = h >> 1;
+        const bool fwd = (h & 1) == 0;
+        const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
+        const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
+        const int nb = (g & 2) ? hi : lo;
+        const unsigned va = unsigned(nb) * site_bytes + rcb;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) psi[st][t][b] = bload(r_in, va + unsigned(t * NC + b) * row_bytes, 0u);
+        // forward: U_mu(y) at parity 0; backward: U_mu(nb) at parity 1 (used as its adjoint)
+        const unsigned gl = fwd ? (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(site)
+                                : (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(nb);
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) lk[st][a][b] = bload(r_g, (gl * NC * NC + unsigned(a * NC + b)) * 4u, 0u);
+    }
+    const bool fwd0 = (g & 1) == 0;  // hop parity is the same in both k steps
+    const size_t fwords = fbytes / 4u;
+    f32x4 d[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) d[c][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        // V[c][b] = U[c][b] forward, conj(U[b][c]) backward, unpacked once
+        float vr[NC][NC], vi[NC][NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                const uint32_t uw = fwd0 ? lk[st][c][b] : lk[st][b][c];
+                vr[c][b] = P::re(uw);
+                vi[c][b] = fwd0 ? P::im(uw) : -P::im(uw);
+            }
+        u32x4 bop[NC];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float pr[NC], pi[NC];
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                pr[b] = P::re(psi[st][t][b]);
+                pi[b] = P::im(psi[st][t][b]);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                float ur = 0.f, ui = 0.f;
+#pragma unroll
+                for (int b = 0; b < NC; ++b) {
+                    ur = fmaf(vr[c][b], pr[b], fmaf(-vi[c][b], pi[b], ur));
+                    ui = fmaf(vr[c][b], pi[b], fmaf(vi[c][b], pr[b], ui));
+                }
+                bop[c][t] = P::pack(ur, ui);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) d[c][rb] = P::mma(A[rb][st], bop[c], d[c][rb]);
+    }
+    // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1
+    if (r < R) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int k = 2 * rb + (g >> 1), sp = 2 * (g & 1) + i;
+                    const size_t off = size_t(k) * fwords + (size_t(site) * 4 + sp) * NC * R + size_t(c) * R + r;
+                    q[off] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
+                }
+    }
+}
+
+template <typename S, int NC>
+__global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
+                                                    const uint32_t *__restrict__ q, const uint32_t *__restrict__ gauge,
+                                                    const Cx<float> *__restrict__ m0, const int *__restrict__ walk1,
+                                                    int Vh, int R, unsigned fbytes, unsigned gauge_bytes) {
+    using P = W<S>;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int m = lane & 15, g = lane >> 4;
+    const int pos = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
+    if (pos >= Vh) return;  // uniform
+    const int r = 16 * int(blockIdx.y) + m;
+    const unsigned row_bytes = unsigned(R) * 4u, site_bytes = 4u * NC * row_bytes;
+    const unsigned rcb = unsigned(min(r, R - 1)) * 4u;
+    const int cb = m >> 1;                 // output colour of A row m
+    const bool bre = (m & 1) == 0;
+    const int cbc = cb < NC ? cb : NC - 1;  // clamped (row never stored)
+    const auto r_g = rsrc(gauge, gauge_bytes);
+    const auto r_in = rsrc(in, fbytes);
+    const auto r_out = rsrc(out, fbytes);
+
+    const int *row = walk1 + long(pos) * 9;
+    const int site = row[0];
+    f32x4 d[4];
+#pragma unroll
+    for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
+    uint32_t qw[2][4][4], lw[2][4];
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        const int h = 4 * st + g, mu = h >> 1;
+        const bool fwd = (h & 1) == 0;
+        const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
+        const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
+        const int nb = (g & 2) ? hi : lo;
+        const unsigned va = unsigned(nb) * site_bytes + rcb;
+        // field h of q at the neighbour: rows (spin, colour c'), c' < NC (padding words are zero)
+        const auto r_q = rsrc(q + size_t(h) * (fbytes / 4u), fbytes);
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp)
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+                qw[st][sp][cc] = cc < NC ? bload(r_q, va + unsigned(sp * NC + cc) * row_bytes, 0u) : 0u;
+        // A word from link V[cb][c']: forward U(x)[cb][c'] (parity 1 at x), backward conj(U(nb)[c'][cb]) (parity 0)
+        const unsigned gl = fwd ? (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(site)
+                                : (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(nb);
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            uint32_t w = 0u;
+            if (cc < NC) {
+                const unsigned idx = fwd ? unsigned(cbc * NC + cc) : unsigned(cc * NC + cbc);
+                const uint32_t uw = bload(r_g, (gl * NC * NC + idx) * 4u, 0u);
+                const float vr = P::re(uw), vi = fwd ? P::im(uw) : -P::im(uw);
+                // real row (Vr, -Vi) for re output, (Vi, Vr) for im output
+                w = bre ? P::pack(vr, -vi) : P::pack(vi, vr);
+            }
+            lw[st][cc] = w;
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+        const u32x4 a = {lw[st][0], lw[st][1], lw[st][2], lw[st][3]};
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) {
+            const u32x4 b = {qw[st][sp][0], qw[st][sp][1], qw[st][sp][2], qw[st][sp][3]};
+            d[sp] = P::mma(a, b, d[sp]);
+        }
+    }
+    // epilogue: lane (m, g) holds rows 4g + e = (colour 2g + e/2, re|im) of RHS r; add M0 psi(x)
+    if (r < R) {
+        const unsigned xb = unsigned(site) * site_bytes + unsigned(r) * 4u;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int c = 2 * g + i;
+            if (c < NC) {
+                float pr[4], pi[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const uint32_t w = bload(r_in, xb + unsigned(t * NC + c) * row_bytes, 0u);
+                    pr[t] = P::re(w);
+                    pi[t] = P::im(w);
+                }
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp) {
+                    float vr = d[sp][2 * i], vi = d[sp][2 * i + 1];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const Cx<float> mm = m0[sp * 4 + t];
+                        vr = fmaf(mm.re, pr[t], fmaf(-mm.im, pi[t], vr));
+                        vi = fmaf(mm.re, pi[t], fmaf(mm.im, pr[t], vi));
+                    }
+                    bstore(r_out, P::pack(vr, vi), xb + unsigned(sp * NC + c) * row_bytes, 0u);
+                }
+            }
+        }
+    }
+}
+
+template <typename S, int NC>
+hipError_t launch_nc(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol, const int *walk,
+                     const Geom &g, int R, hipStream_t s) {
+    const unsigned fbytes = unsigned(size_t(g.Vh) * 4 * NC * R * 4);
+    const unsigned gbytes = unsigned(size_t(g.Vh) * 8 * NC * NC * 4);
+    const dim3 grid(div_up(g.Vh, 4), div_up(R, 16)), block(256);
+    // the A fragments live after the 8 q fields
+    auto *frag = reinterpret_cast<u32x4 *>(static_cast<char *>(q) + 8 * size_t(fbytes));
+    hipLaunchKernelGGL((k_lgnn_frags<S>), dim3(2), dim3(256), 0, s, frag, mcol);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_lgnn_mix<S, NC>), grid, block, 0, s, static_cast<uint32_t *>(q),
+                       static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(gauge), frag, walk, g.Vh, R,
+                       fbytes, gbytes);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_lgnn_gather<S, NC>), grid, block, 0, s, static_cast<uint32_t *>(out),
+                       static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(q),
+                       static_cast<const uint32_t *>(gauge), mcol + 1024, walk + size_t(g.Vh) * 9, g.Vh, R, fbytes,
+                       gbytes);
+    return hipGetLastError();
+}
+
+template <typename S>
+hipError_t launch_s(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol, const int *walk,
+                    const Geom &g, int Nc, int R, hipStream_t s) {
+    switch (Nc) {
+        case 2: return launch_nc<S, 2>(out, in, gauge, q, mcol, walk, g, R, s);
+        case 3: return launch_nc<S, 3>(out, in, gauge, q, mcol, walk, g, R, s);
+        case 4: return launch_nc<S, 4>(out, in, gauge, q, mcol, walk, g, R, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
+                            const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s) {
+    if (Nc < 2 || Nc > 4 || R < 1) return hipErrorInvalidValue;
+    if (double(g.Vh) * 16.0 * Nc * R >= 2147483647.0 || double(g.Vh) * 32.0 * Nc * Nc >= 2147483647.0)
+        return hipErrorInvalidValue;
+    if (prec == kHalf) return launch_s<__half>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
+    if (prec == kBF16) return launch_s<__hip_bfloat16>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace qcu

@@ -45,7 +45,9 @@ def _abi_weights(net):
     return net.W1[:, 0].copy(), net.W2[0].copy()
 
 
-@pytest.mark.parametrize("prec,tol", [(2, RTOL_FWD), (1, RTOL_FWD), (0, 2e-3), (4, 1.5e-2)])
+# FP16 / BF16 storage runs the matrix-core path: inputs, M and the mixed hop
+# fields are rounded to 16 bits (several roundings of 2^-11 / 2^-8 relative)
+@pytest.mark.parametrize("prec,tol", [(2, RTOL_FWD), (1, RTOL_FWD), (0, 4e-3), (4, 2.5e-2)])
 def test_forward_matches_oracle(prec, tol):
     _case("lgnn forward", prec)
     gauge = fields.gauge_field(L, Nc, 71, kind="haar")
@@ -64,6 +66,31 @@ def test_forward_matches_oracle(prec, tol):
     err = _rel(got, ref.reshape(-1))
     print(f"precision {prec}: forward relative error {err:.2e}")
     assert err < tol
+    api.finalizeQcu()
+    del d_g
+
+
+@pytest.mark.parametrize("Nc_,R_,prec", [(3, 17, 0), (2, 16, 0), (4, 5, 4), (3, 32, 4)])
+def test_forward_matrix_core_path_shapes(Nc_, R_, prec):
+    """FP16/BF16 application runs on the matrix-core kernels (transport + MFMA
+    mixing, MFMA gather); RHS chunks beyond 16, partial chunks and Nc 2..4
+    against the FP64 oracle on inputs rounded to the storage precision."""
+    _case("lgnn forward mfma", Nc_, R_, prec)
+    gauge = fields.gauge_field(L, Nc_, 171 + Nc_, kind="haar")
+    d_g = _setup(L, Nc_, R_, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    psi = _odd_field(172 + R_, (Vh, 4, Nc_, R_))
+    net = olgnn.Net.random(8, 173, scale=0.3)
+    lgnn.set_weights(*_abi_weights(net))
+    d_in = api.DeviceArray.from_complex(psi.reshape(-1), prec)
+    d_out = api.DeviceArray(api.storage_bytes(psi.size, prec))
+    lgnn.apply(d_out, d_in, prec)
+    got = d_out.complex(psi.size, prec)
+    psi_s = api.from_storage(api.to_storage(psi.reshape(-1), prec), prec).reshape(psi.shape)
+    ref, _ = net.forward(olgnn.Transport(gauge, L, Nc_), psi_s, L, Nc_, R_)
+    err = _rel(got, ref.reshape(-1))
+    print(f"Nc {Nc_} R {R_} precision {prec}: forward relative error {err:.2e}")
+    assert err < (4e-3 if prec == 0 else 2.5e-2)
     api.finalizeQcu()
     del d_g
 
