@@ -146,50 +146,51 @@ __global__ __launch_bounds__(256) void k_gather8(CO *__restrict__ out, const CS 
 
 // Partial sums of G[(k,s)][(k',s')] = sum_j a_{k^1}[s](j) conj(b_k'[s'](j)) over
 // columns j = (even site, colour, rhs); a = transported dL/dphi*, b = u.
-// Block partials in FP64; thread t owns entries 4t .. 4t+3.
+// Block partials in FP64; thread t owns entries 4t .. 4t+3.  Each block walks
+// a contiguous range of sites; per site the (colour, rhs) columns are one
+// contiguous run of Nc*R entries in every (field, spin) row.
 constexpr int kGramTile = 32;
 __global__ __launch_bounds__(256) void k_gram(C64 *__restrict__ part, const C32 *__restrict__ a,
-                                             const C32 *__restrict__ b, long ncol, int Nc, int R, long fsize) {
+                                             const C32 *__restrict__ b, int Vh, int ncr, long fsize) {
     __shared__ C32 sa[32][kGramTile];
     __shared__ C32 sb[32][kGramTile];
     const int t = threadIdx.x;
     const int i = t >> 3;            // row (k, s)
     const int j0 = (t & 7) * 4;      // first of 4 columns (k', s')
     C64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    const long per_block = (ncol + gridDim.x - 1) / gridDim.x;
-    const long lo = per_block * blockIdx.x;
-    const long hi = lo + per_block < ncol ? lo + per_block : ncol;
-    for (long base = lo; base < hi; base += kGramTile) {
-        // stage kGramTile columns of both 32-vectors
-        for (int e = t; e < 32 * kGramTile; e += blockDim.x) {
-            const int row = e / kGramTile, col = e % kGramTile;
-            const long j = base + col;
-            C32 va = {0.f, 0.f}, vb = {0.f, 0.f};
-            if (j < hi) {
-                const long r = j % R;
-                const long rest = j / R;
-                const long c = rest % Nc;
-                const long site = rest / Nc;
-                const int k = row >> 2, s = row & 3;
-                const long off = ((site * 4 + s) * Nc + c) * R + r;
-                va = a[(k ^ 1) * fsize + off];
-                vb = b[k * fsize + off];
+    const int per_block = (Vh + int(gridDim.x) - 1) / int(gridDim.x);
+    const int lo = per_block * int(blockIdx.x);
+    const int hi = lo + per_block < Vh ? lo + per_block : Vh;
+    for (int site = lo; site < hi; ++site) {
+        const long sbase = long(site) * 4 * ncr;
+        for (int cr0 = 0; cr0 < ncr; cr0 += kGramTile) {
+            // stage kGramTile columns of both 32-vectors
+            for (int e = t; e < 32 * kGramTile; e += blockDim.x) {
+                const int row = e / kGramTile, col = e % kGramTile;
+                const int cr = cr0 + col;
+                C32 va = {0.f, 0.f}, vb = {0.f, 0.f};
+                if (cr < ncr) {
+                    const int k = row >> 2, sp = row & 3;
+                    const long off = sbase + long(sp) * ncr + cr;
+                    va = a[(k ^ 1) * fsize + off];
+                    vb = b[k * fsize + off];
+                }
+                sa[row][col] = va;
+                sb[row][col] = vb;
             }
-            sa[row][col] = va;
-            sb[row][col] = vb;
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll 4
-        for (int col = 0; col < kGramTile; ++col) {
-            const C32 x = sa[i][col];
+            for (int col = 0; col < kGramTile; ++col) {
+                const C32 x = sa[i][col];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const C32 y = sb[j0 + q][col];
-                acc[q].re += double(x.re) * y.re + double(x.im) * y.im;
-                acc[q].im += double(x.im) * y.re - double(x.re) * y.im;
+                for (int q = 0; q < 4; ++q) {
+                    const C32 y = sb[j0 + q][col];
+                    acc[q].re += double(x.re) * y.re + double(x.im) * y.im;
+                    acc[q].im += double(x.im) * y.re - double(x.re) * y.im;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) part[long(blockIdx.x) * 1040 + i * 32 + j0 + q] = acc[q];
@@ -240,16 +241,31 @@ __global__ __launch_bounds__(256) void k_gram0(C64 *__restrict__ part, const C32
 //   dL/dM* = G, dL/dW2[o][h]* = sum_h' G[h][h'] W1[o][h']^+, dL/dW1[o][h']* = sum_h W2[o][h]^+ G[h][h']
 // (h = 0 pairs with M0).  The stored gradient is 2 dL/dW* = dL/dRe W + i dL/dIm W,
 // accumulated over batches.
-__global__ void k_chain(C64 *g1, C64 *g2, const C64 *w1, const C64 *w2, const C64 *part, int nblk, int C) {
-    __shared__ C64 G[1040];
-    for (int e = threadIdx.x; e < 1040; e += blockDim.x) {
-        C64 s = {0.0, 0.0};
-        for (int b = 0; b < nblk; ++b) {
-            s.re += part[long(b) * 1040 + e].re;
-            s.im += part[long(b) * 1040 + e].im;
-        }
-        G[e] = s;
+// G[e] = sum over the block partials of entry e (fixed order: strided lanes, then a tree)
+__global__ __launch_bounds__(256) void k_sum_partials(C64 *__restrict__ G, const C64 *__restrict__ part, int nblk) {
+    __shared__ double sr[256], si[256];
+    const int e = blockIdx.x;
+    double ar = 0.0, ai = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        ar += part[long(b) * 1040 + e].re;
+        ai += part[long(b) * 1040 + e].im;
     }
+    sr[threadIdx.x] = ar;
+    si[threadIdx.x] = ai;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+            sr[threadIdx.x] += sr[threadIdx.x + w];
+            si[threadIdx.x] += si[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) G[e] = {sr[0], si[0]};
+}
+
+__global__ void k_chain(C64 *g1, C64 *g2, const C64 *w1, const C64 *w2, const C64 *Gsum, int C) {
+    __shared__ C64 G[1040];
+    for (int e = threadIdx.x; e < 1040; e += blockDim.x) G[e] = Gsum[e];
     __syncthreads();
     // entry (o, h, s, t) of W: flattened ((o * 9 + h) * 4 + s) * 4 + t
     const int total = C * 9 * 16;
@@ -408,7 +424,7 @@ hipError_t LgnnPreconditioner::ensure_work32(hipStream_t) {
     phi_ = dmal(half32, err);
     dlt_ = dmal(half32, err);
     gram_blocks_ = 512;
-    gm_ = static_cast<C64 *>(dmal(sizeof(C64) * 1040 * size_t(gram_blocks_), err));
+    gm_ = static_cast<C64 *>(dmal(sizeof(C64) * 1040 * size_t(gram_blocks_ + 1), err));
     if (err != hipSuccess) set_error(5, "learned preconditioner: allocation failed (%s)", hipGetErrorString(err));
     return err;
 }
@@ -473,6 +489,7 @@ hipError_t LgnnPreconditioner::collapse(hipStream_t s) {
 
 hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int out_prec, hipStream_t s) {
     QCU_HIP_TRY(ensure_work32(s));
+    if (!in || !out || !u_ || !q_) return hipErrorInvalidValue;
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
     const int spb = R_ >= 256 ? 1 : 256 / R_;
@@ -558,6 +575,8 @@ hipError_t LgnnPreconditioner::zero_grad(hipStream_t s) {
 hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, double *loss_host, hipStream_t s) {
     if (!ready()) return hipErrorInvalidValue;
     QCU_HIP_TRY(ensure_fields(s));
+    // the FP32 work fields (phi_, dlt_, u_, q_, gm_) must exist before their pointers are passed on
+    QCU_HIP_TRY(ensure_work32(s));
     const long N = n_ * R_;
     // forward in FP32 (keeps u for the gradient)
     QCU_HIP_TRY(forward(in, kSingle, phi_, kSingle, s));
@@ -593,14 +612,18 @@ hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, dou
     const long ncol = long(g_.Vh) * Nc_ * R_;
     const long fsize = n_ * R_;
     hipLaunchKernelGGL(k_gram, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(q_),
-                       static_cast<const C32 *>(u_), ncol, Nc_, R_, fsize);
+                       static_cast<const C32 *>(u_), g_.Vh, Nc_ * R_, fsize);
     QCU_HIP_TRY(hipGetLastError());
     // the input itself is needed in FP32 for the M0 gradient
     const C32 *inf = static_cast<const C32 *>(in);
     hipLaunchKernelGGL(k_gram0, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(dlt_), inf, ncol,
                        Nc_, R_);
     QCU_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_chain, dim3(1), dim3(256), 0, s, g1_, g2_, w1_, w2_, gm_, gram_blocks_, C_);
+    // partial sums land after the per-block partials
+    C64 *gsum = gm_ + 1040 * size_t(gram_blocks_);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1040), dim3(256), 0, s, gsum, gm_, gram_blocks_);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_chain, dim3(1), dim3(256), 0, s, g1_, g2_, w1_, w2_, gsum, C_);
     return hipGetLastError();
 }
 

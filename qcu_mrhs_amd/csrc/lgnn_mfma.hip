@@ -337,7 +337,7 @@ hipError_t launch_s(void *out, const void *in, const void *gauge, void *q, const
 
 hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
                             const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s) {
-    if (Nc < 2 || Nc > 4 || R < 1) return hipErrorInvalidValue;
+    if (Nc < 2 || Nc > 4 || R < 1 || !out || !in || !gauge || !q || !mcol || !walk) return hipErrorInvalidValue;
     if (double(g.Vh) * 16.0 * Nc * R >= 2147483647.0 || double(g.Vh) * 32.0 * Nc * Nc >= 2147483647.0)
         return hipErrorInvalidValue;
     if (prec == kHalf) return launch_s<__half>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
