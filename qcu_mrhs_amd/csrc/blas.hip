@@ -85,12 +85,10 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
     if (active) {
         auto loc = op.local(rhs);
         long e = g;
-        // four entries per trip keep more loads in flight (same summation order as one per trip)
-        for (; e + 3 * stride < N; e += 4 * stride) {
-            op.elem(e, loc, acc);
-            op.elem(e + stride, loc, acc);
-            op.elem(e + 2 * stride, loc, acc);
-            op.elem(e + 3 * stride, loc, acc);
+        // eight entries per trip keep more loads in flight (same summation order as one per trip)
+        for (; e + 7 * stride < N; e += 8 * stride) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) op.elem(e + u * stride, loc, acc);
         }
         for (; e < N; e += stride) op.elem(e, loc, acc);
     }
@@ -261,9 +259,11 @@ struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
         const Cx<T> xv = load_cx<T>(x + e) + l.al * load_cx<T>(p + e) + l.om * shv;
         store_cx(x + e, xv);
         const Cx<T> rv = sv - l.om * load_cx<T>(t + e);
-        store_cx(r + e, rv);
-        // reductions use the stored (rounded) residual
-        const Cx<double> rd = load_cx<double>(r + e);
+        // reductions use the stored (rounded) residual: round in registers, not through memory
+        CS rs;
+        store_cx(&rs, rv);
+        r[e] = rs;
+        const Cx<double> rd = load_cx<double>(&rs);
         const Cx<double> u = load_cx<double>(r0 + e);
         acc[0] = fma(rd.re, rd.re, fma(rd.im, rd.im, acc[0]));
         acc[1] = fma(u.re, rd.re, fma(u.im, rd.im, acc[1]));

@@ -8,8 +8,8 @@
 // (hop, spin) and M0 a 4 x 4 spin matrix.
 //
 // k_lgnn_mix: one wavefront per (even site, 16-RHS chunk).  Lane (m, g)
-// (m = RHS, g = lane group) transports, for one colour at a time, the 4 spins
-// of hop k' = 4 st + g on VALU (FP32) and packs them as the MFMA B operand
+// (m = RHS, g = lane group) transports the 4 spins of hop k' = 4 st + g on
+// VALU (v_dot2 on 16-bit pairs, FP32 accumulation) and packs them as the MFMA B operand
 // (k index = (k', spin, re|im)); the real 2x2 embedding of M is the A operand
 // (rows = (k, s, re|im)), kept in registers for the whole wavefront.  8 MFMAs
 // (4 row blocks x 2 k steps) per colour give all 32 q entries of that colour.
@@ -43,6 +43,10 @@ template <> struct W<__half> {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b),
                                                       c, 0, 0, 0);
     }
+    // c + a.lo * b.lo + a.hi * b.hi (FP32 accumulation)
+    __device__ __forceinline__ static float dot(uint32_t a, uint32_t b, float c) {
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(f16x2, a), __builtin_bit_cast(f16x2, b), c, false);
+    }
 };
 template <> struct W<__hip_bfloat16> {
     __device__ __forceinline__ static float re(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
@@ -54,6 +58,10 @@ template <> struct W<__hip_bfloat16> {
     __device__ __forceinline__ static f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+    __device__ __forceinline__ static float dot(uint32_t a, uint32_t b, float c) {
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a), __builtin_bit_cast(bf16x2, b), c,
+                                               false);
     }
 };
 
@@ -152,55 +160,56 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) d[c][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // word permutations: swap the two 16-bit halves; sign of the high / low half
+    constexpr uint32_t kSwap = 0x01000302u, kSignHi = 0x80000000u, kSignLo = 0x00008000u;
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
-        // V[c][b] = U[c][b] forward, conj(U[b][c]) backward, unpacked once
-        float vr[NC][NC], vi[NC][NC];
+        // u = V psi with V[c][b] = U[c][b] forward, conj(U[b][c]) backward, as two dot2 per term:
+        //   forward:  re = (Ur, -Ui).(pr, pi), im = (Ui, Ur).(pr, pi)
+        //   backward: re = (Ur,  Ui).(pr, pi), im = (-Ui, Ur).(pr, pi)
+        uint32_t la[NC][NC], lb[NC][NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int b = 0; b < NC; ++b) {
                 const uint32_t uw = fwd0 ? lk[st][c][b] : lk[st][b][c];
-                vr[c][b] = P::re(uw);
-                vi[c][b] = fwd0 ? P::im(uw) : -P::im(uw);
+                const uint32_t sw = __builtin_amdgcn_perm(uw, uw, kSwap);
+                la[c][b] = fwd0 ? (uw ^ kSignHi) : uw;
+                lb[c][b] = fwd0 ? sw : (sw ^ kSignLo);
             }
         u32x4 bop[NC];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            float pr[NC], pi[NC];
-#pragma unroll
-            for (int b = 0; b < NC; ++b) {
-                pr[b] = P::re(psi[st][t][b]);
-                pi[b] = P::im(psi[st][t][b]);
-            }
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 float ur = 0.f, ui = 0.f;
 #pragma unroll
                 for (int b = 0; b < NC; ++b) {
-                    ur = fmaf(vr[c][b], pr[b], fmaf(-vi[c][b], pi[b], ur));
-                    ui = fmaf(vr[c][b], pi[b], fmaf(vi[c][b], pr[b], ui));
+                    ur = P::dot(la[c][b], psi[st][t][b], ur);
+                    ui = P::dot(lb[c][b], psi[st][t][b], ui);
                 }
                 bop[c][t] = P::pack(ur, ui);
             }
-        }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) d[c][rb] = P::mma(A[rb][st], bop[c], d[c][rb]);
     }
-    // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1
+    // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
+    // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
     if (r < R) {
+        const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
+                               unsigned(2 * (g & 1)) * NC * row_bytes + unsigned(r) * 4u;
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+        for (int rb = 0; rb < 4; ++rb) {
+            const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
-                    const int k = 2 * rb + (g >> 1), sp = 2 * (g & 1) + i;
-                    const size_t off = size_t(k) * fwords + (size_t(site) * 4 + sp) * NC * R + size_t(c) * R + r;
-                    q[off] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
-                }
+                for (int i = 0; i < 2; ++i)
+                    bstore(r_q, P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]), lbase,
+                           unsigned(i * NC + c) * row_bytes);
+        }
     }
 }
 
@@ -338,7 +347,8 @@ hipError_t launch_s(void *out, const void *in, const void *gauge, void *q, const
 hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
                             const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s) {
     if (Nc < 2 || Nc > 4 || R < 1 || !out || !in || !gauge || !q || !mcol || !walk) return hipErrorInvalidValue;
-    if (double(g.Vh) * 16.0 * Nc * R >= 2147483647.0 || double(g.Vh) * 32.0 * Nc * Nc >= 2147483647.0)
+    // two q fields share one buffer resource: 2 x (Vh 4 Nc R words) must stay below 2^31 bytes
+    if (double(g.Vh) * 32.0 * Nc * R >= 2147483647.0 || double(g.Vh) * 32.0 * Nc * Nc >= 2147483647.0)
         return hipErrorInvalidValue;
     if (prec == kHalf) return launch_s<__half>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
     if (prec == kBF16) return launch_s<__hip_bfloat16>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
