@@ -228,7 +228,8 @@ def main():
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--methods", default="half,bf16,single",
                     help="mixed inner precisions to time (headline = fastest converged)")
-    ap.add_argument("--delta", type=float, default=0.1)
+    ap.add_argument("--delta", type=float, default=0.0,
+                    help="reliable-update threshold (0 = the library default for the inner precision)")
     ap.add_argument("--max-iter", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -125,7 +125,8 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info);
 
 /* Solver options.  inner_precision: precision of the inner (sloppy) solve,
  * overriding initGridSize's dslashFloatPrecision; delta: reliable-update
- * threshold (update when ||r|| < delta * max ||r|| since the last update);
+ * threshold (update when ||r|| < delta * max ||r|| since the last update;
+ * 0 = default for the inner precision: 1e-3 FP32, 5e-3 FP16, 1e-2 BF16);
  * use_preconditioner: 1 = apply the loaded learned right preconditioner; if
  * that solve breaks down, diverges or does not converge, plain BiCGStab
  * solves the same system again and QcuSolveInfo.fallback is set. */

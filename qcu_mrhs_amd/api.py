@@ -227,7 +227,7 @@ def invert_groups(groups, max_iteration, tol, raise_on_nonconvergence=False):
     return [{k: getattr(info[g], k) for k, _ in QcuGroupInfo._fields_} for g in range(n)]
 
 
-def set_solver_options(inner_precision=2, delta=0.1, use_preconditioner=False):
+def set_solver_options(inner_precision=2, delta=0.0, use_preconditioner=False):
     check(lib().qcuSetSolverOptions(int(inner_precision), float(delta), int(bool(use_preconditioner))))
 
 

@@ -26,11 +26,9 @@ struct AxpyArgs {
 };
 hipError_t launch_axpy(const AxpyArgs &a, hipStream_t s);
 
-// Reduction workspace: per-block partial sums + one completion counter per
-// reduction slot (zero-initialised once; the last block resets its counter).
+// Reduction workspace: per-block partial sums, one region per reduction slot.
 struct ReduceWork {
-    double *partials = nullptr;        // [max_blocks][R][4]
-    unsigned int *counters = nullptr;  // [kReduceSlots]
+    double *partials = nullptr;        // [kReduceSlots][max_blocks][R][4]
     int max_blocks = 0;
 };
 constexpr int kReduceSlots = 16;
@@ -98,12 +96,12 @@ hipError_t launch_scalar_fill(C64 *dst, C64 value, int count, hipStream_t s);
 // ---------------------------------------------------------------------------
 // Batched groups (SegTab layout, qcu_common.h): one launch covers every group
 // (grid.y = group); groups whose done flag is set are skipped.  Scalars use
-// slot stride tab.Rtot.  Reductions keep partials per group and one
-// completion counter per (slot, group).
+// slot stride tab.Rtot.  Reductions keep partials per group (the slot
+// argument is accepted for symmetry with the single-group calls; a batch's
+// reductions run in stream order).
 // ---------------------------------------------------------------------------
 struct SegWork {
     double *partials = nullptr;        // [G][max_blocks][Rmax][4]
-    unsigned int *counters = nullptr;  // [kReduceSlots][kMaxGroups]
     int max_blocks = 0, G = 0, Rmax = 0;
 };
 hipError_t seg_work_alloc(SegWork &w, int G, int Rmax);

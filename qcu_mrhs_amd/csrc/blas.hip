@@ -5,12 +5,12 @@
 // RHS index is fixed for its whole grid-stride loop and consecutive lanes
 // touch consecutive complex numbers of the [n][R] layout (coalesced).
 //
-// Reductions are single pass: each block reduces its lanes per RHS through
-// LDS in a fixed order and writes R partials; the last block to finish
-// (device-scope counter, release/acquire fences) sums the partials in block
-// order and runs the op's finisher (e.g. alpha = rho / <r0, v>) on device,
-// so no host round trip is needed.  Results are bit-reproducible for a given
-// (n, R).  Accumulation is FP64 for every storage precision.
+// Reductions read their vectors once: each block reduces its lanes per RHS
+// through LDS in a fixed order and writes R partials, and a one-block kernel
+// sums the partials in block order and runs the op's finisher (e.g.
+// alpha = rho / <r0, v>) on device, so no host round trip is needed.  Results
+// are bit-reproducible for a given (n, R).  Accumulation is FP64 for every
+// storage precision.
 #include "blas.h"
 
 namespace qcu {
@@ -69,34 +69,35 @@ __global__ __launch_bounds__(256) void k_axpy(CS *__restrict__ res, const CS *__
 }
 
 // ------------------------------------------------------------- reduction
-// Returns true in the block that ran the finisher (the last one to finish).
+// Two kernels per reduction: every block reduces its lanes per RHS through
+// LDS in a fixed order and writes R partials; a one-block kernel then sums the
+// partials in block order and runs the op's finisher.  The kernel boundary
+// orders the partial writes before their reads: no device-scope fences or
+// atomics inside the hot kernel (on a multi-XCD GPU a device-scope release
+// writes back the XCD's L2, which made a per-block fence the dominant cost).
 template <class Op>
-__device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double *__restrict__ partials,
-                                            unsigned int *__restrict__ counter) {
+__device__ __forceinline__ void reduce_partial(const Op &op, long N, int R, double *__restrict__ partials) {
     constexpr int NA = Op::NA;
     __shared__ double buf[NA][kReduceThreads];
-    __shared__ bool am_last;
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
     long g, stride;
     int rhs;
-    const bool active = rhs_lane(R, g, stride, rhs);
-    if (active) {
+    if (rhs_lane(R, g, stride, rhs)) {
         auto loc = op.local(rhs);
         long e = g;
-        // eight entries per trip keep more loads in flight (same summation order as one per trip)
-        for (; e + 7 * stride < N; e += 8 * stride) {
+        // four entries per trip keep more loads in flight (same summation order as one per trip)
+        for (; e + 3 * stride < N; e += 4 * stride) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) op.elem(e + u * stride, loc, acc);
+            for (int u = 0; u < 4; ++u) op.elem(e + u * stride, loc, acc);
         }
         for (; e < N; e += stride) op.elem(e, loc, acc);
     }
 #pragma unroll
     for (int k = 0; k < NA; ++k) buf[k][threadIdx.x] = acc[k];
     __syncthreads();
-    const int per_block = (kReduceThreads / R) * R;
-    const int nch = per_block / R;
+    const int nch = kReduceThreads / R;
     if (int(threadIdx.x) < R) {
         double s[NA];
 #pragma unroll
@@ -107,25 +108,24 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
 #pragma unroll
         for (int k = 0; k < NA; ++k) partials[(long(blockIdx.x) * R + threadIdx.x) * 4 + k] = s[k];
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int prev = atomicAdd(counter, 1u);
-        am_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    // last block: sum partials in block order
+}
+
+// One block: sum `nblk` block partials per RHS in a fixed order, then finish.
+template <class Op>
+__device__ __forceinline__ void reduce_finish(const Op &op, int nblk, int R, const double *__restrict__ partials) {
+    constexpr int NA = Op::NA;
+    __shared__ double buf[NA][kReduceThreads];
     const int t = threadIdx.x;
+    const int per_block = (kReduceThreads / R) * R;
+    const int nch = per_block / R;
+    double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
     if (t < per_block) {
         const int r = t % R, ch = t / R;
-        // four independent partial loads per trip (latency of the L2 reads overlaps);
-        // fixed order: bb, bb + nch, bb + 2 nch, bb + 3 nch, then the next four
+        // four independent partial loads per trip; fixed order bb, bb + nch, bb + 2 nch, bb + 3 nch
         int bb = ch;
-        for (; bb + 3 * nch < int(gridDim.x); bb += 4 * nch) {
+        for (; bb + 3 * nch < nblk; bb += 4 * nch) {
             double v[4][NA];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -136,11 +136,10 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
 #pragma unroll
                 for (int k = 0; k < NA; ++k) acc[k] += v[u][k];
         }
-        for (; bb < int(gridDim.x); bb += nch)
+        for (; bb < nblk; bb += nch)
 #pragma unroll
             for (int k = 0; k < NA; ++k) acc[k] += partials[(long(bb) * R + r) * 4 + k];
     }
-    __syncthreads();
 #pragma unroll
     for (int k = 0; k < NA; ++k) buf[k][t] = acc[k];
     __syncthreads();
@@ -153,15 +152,17 @@ __device__ __forceinline__ bool reduce_body(const Op &op, long N, int R, double 
             for (int k = 0; k < NA; ++k) tot[k] += buf[k][j * R + t];
         op.finish(t, tot);
     }
-    if (t == 0) *counter = 0u;
-    return true;
 }
 
 template <class Op>
-__global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R,
-                                                          double *__restrict__ partials,
-                                                          unsigned int *__restrict__ counter) {
-    (void)reduce_body(op, N, R, partials, counter);
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(Op op, long N, int R, double *__restrict__ partials) {
+    reduce_partial(op, N, R, partials);
+}
+
+template <class Op>
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_finish(Op op, int nblk, int R,
+                                                                 const double *__restrict__ partials) {
+    reduce_finish(op, nblk, R, partials);
 }
 
 template <typename CS>
@@ -347,28 +348,36 @@ __device__ __forceinline__ bool seg_group(const SegTab &tab, int &grp, int &R, l
     return true;
 }
 
-// Reduction over every group; with GROUP_DONE the finishing block of a group
-// also counts the iteration and raises the group's done flag once every RHS
-// has ||r||^2 <= st[kBnorm2] (the squared stopping threshold).
-template <class Op, bool GROUP_DONE>
+// Reduction over every group (grid.y = group), partial pass.
+template <class Op>
 __global__ __launch_bounds__(kReduceThreads) void k_reduce_seg(Op op, SegTab tab, double *__restrict__ partials,
-                                                              unsigned int *__restrict__ counters, int max_blocks,
-                                                              const C64 *st, int *done, int *iters) {
+                                                              int max_blocks) {
     int grp, R, base;
     long off;
     if (!seg_group(tab, grp, R, off, base)) return;
-    const bool last = reduce_body(op.shifted(off, base), tab.n * R, R,
-                                  partials + size_t(grp) * max_blocks * tab.Rmax * 4, counters + grp);
+    reduce_partial(op.shifted(off, base), tab.n * R, R, partials + size_t(grp) * max_blocks * tab.Rmax * 4);
+}
+
+// Finishing pass, one block per group; with GROUP_DONE it also counts the
+// iteration and raises the group's done flag once every RHS has
+// ||r||^2 <= st[kBnorm2] (the squared stopping threshold).
+template <class Op, bool GROUP_DONE>
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_seg_finish(Op op, SegTab tab,
+                                                                     const double *__restrict__ partials,
+                                                                     int max_blocks, int nblk, const C64 *st,
+                                                                     int *done, int *iters) {
+    int grp, R, base;
+    long off;
+    if (!seg_group(tab, grp, R, off, base)) return;
+    reduce_finish(op.shifted(off, base), nblk, R, partials + size_t(grp) * max_blocks * tab.Rmax * 4);
     if constexpr (GROUP_DONE) {
-        if (last) {
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                bool all = true;
-                for (int r = 0; r < R; ++r)
-                    all = all && st[kRnorm2 * tab.Rtot + base + r].re <= st[kBnorm2 * tab.Rtot + base + r].re;
-                iters[grp] += 1;
-                if (all) done[grp] = 1;
-            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            bool all = true;
+            for (int r = 0; r < R; ++r)
+                all = all && st[kRnorm2 * tab.Rtot + base + r].re <= st[kBnorm2 * tab.Rtot + base + r].re;
+            iters[grp] += 1;
+            if (all) done[grp] = 1;
         }
     }
 }
@@ -467,10 +476,12 @@ inline int stream_blocks(long N, int R) {
 
 template <class Op>
 hipError_t run_reduce(const Op &op, long n, int R, ReduceWork &w, int slot, hipStream_t s) {
-    if (R < 1 || R > kReduceThreads) return hipErrorInvalidValue;
+    if (R < 1 || R > kReduceThreads || slot < 0 || slot >= kReduceSlots) return hipErrorInvalidValue;
     const int nb = reduce_blocks(n, R, w.max_blocks);
-    hipLaunchKernelGGL(k_reduce<Op>, dim3(nb), dim3(kReduceThreads), 0, s, op, n * R, R,
-                       w.partials, w.counters + slot);
+    double *part = w.partials + size_t(slot) * w.max_blocks * R * 4;
+    hipLaunchKernelGGL(k_reduce<Op>, dim3(nb), dim3(kReduceThreads), 0, s, op, n * R, R, part);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_finish<Op>, dim3(1), dim3(kReduceThreads), 0, s, op, nb, R, part);
     return hipGetLastError();
 }
 
@@ -487,16 +498,13 @@ int reduce_blocks(long n, int R, int max_blocks) {
 
 hipError_t reduce_work_alloc(ReduceWork &w, int R) {
     w.max_blocks = 1024;
-    QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R));
-    QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots));
-    QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots));
-    QCU_HIP_TRY(hipDeviceSynchronize());
+    // one partials region per slot: reductions on different slots may be in flight together
+    QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R * kReduceSlots));
     return hipSuccess;
 }
 
 void reduce_work_free(ReduceWork &w) {
     if (w.partials) (void)hipFree(w.partials);
-    if (w.counters) (void)hipFree(w.counters);
     w = ReduceWork{};
 }
 
@@ -624,15 +632,11 @@ hipError_t seg_work_alloc(SegWork &w, int G, int Rmax) {
     w.G = G;
     w.Rmax = Rmax;
     QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * Rmax * G));
-    QCU_HIP_TRY(hipMalloc(&w.counters, sizeof(unsigned int) * kReduceSlots * kMaxGroups));
-    QCU_HIP_TRY(hipMemset(w.counters, 0, sizeof(unsigned int) * kReduceSlots * kMaxGroups));
-    QCU_HIP_TRY(hipDeviceSynchronize());
     return hipSuccess;
 }
 
 void seg_work_free(SegWork &w) {
     if (w.partials) (void)hipFree(w.partials);
-    if (w.counters) (void)hipFree(w.counters);
     w = SegWork{};
 }
 
@@ -646,12 +650,15 @@ bool seg_ok(const SegTab &tab, const SegWork *w) {
 }
 
 template <class Op, bool GD = false>
-hipError_t run_seg_reduce(const Op &op, const SegTab &tab, SegWork &w, int slot, hipStream_t s,
-                          const C64 *st = nullptr, int *done = nullptr, int *iters = nullptr) {
-    if (!seg_ok(tab, &w) || slot < 0 || slot >= kReduceSlots) return hipErrorInvalidValue;
+hipError_t run_seg_reduce(const Op &op, const SegTab &tab, SegWork &w, hipStream_t s, const C64 *st = nullptr,
+                          int *done = nullptr, int *iters = nullptr) {
+    if (!seg_ok(tab, &w)) return hipErrorInvalidValue;
     const int nb = reduce_blocks(tab.n, tab.Rmax, w.max_blocks);
-    hipLaunchKernelGGL((k_reduce_seg<Op, GD>), dim3(nb, tab.G), dim3(kReduceThreads), 0, s, op, tab, w.partials,
-                       w.counters + slot * kMaxGroups, w.max_blocks, st, done, iters);
+    hipLaunchKernelGGL(k_reduce_seg<Op>, dim3(nb, tab.G), dim3(kReduceThreads), 0, s, op, tab, w.partials,
+                       w.max_blocks);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL((k_reduce_seg_finish<Op, GD>), dim3(1, tab.G), dim3(kReduceThreads), 0, s, op, tab,
+                       w.partials, w.max_blocks, nb, st, done, iters);
     return hipGetLastError();
 }
 
@@ -664,7 +671,8 @@ hipError_t seg_dotc(C64 *res, const void *a, const void *b, int prec, const SegT
     QCU_PREC_DISPATCH(prec, {
         using CS = typename CxStore<typename PrecT<P>::S>::type;
         DotOp<CS> op{static_cast<const CS *>(a), static_cast<const CS *>(b), res};
-        return run_seg_reduce(op, tab, w, slot, s);
+        (void)slot;
+        return run_seg_reduce(op, tab, w, s);
     })
 }
 
@@ -673,7 +681,8 @@ hipError_t seg_norm(double *res, const void *a, int prec, bool take_sqrt, const 
     QCU_PREC_DISPATCH(prec, {
         using CS = typename CxStore<typename PrecT<P>::S>::type;
         NormOp<CS> op{static_cast<const CS *>(a), res, take_sqrt};
-        return run_seg_reduce(op, tab, w, slot, s);
+        (void)slot;
+        return run_seg_reduce(op, tab, w, s);
     })
 }
 
@@ -682,7 +691,7 @@ hipError_t seg_step_alpha(C64 *st, const void *r0, const void *v, int prec, cons
     QCU_PREC_DISPATCH(prec, {
         using CS = typename CxStore<typename PrecT<P>::S>::type;
         AlphaOp<CS> op{static_cast<const CS *>(r0), static_cast<const CS *>(v), st, tab.Rtot};
-        return run_seg_reduce(op, tab, w, 0, s);
+        return run_seg_reduce(op, tab, w, s);
     })
 }
 
@@ -691,7 +700,7 @@ hipError_t seg_step_omega(C64 *st, const void *t, const void *sv, int prec, cons
     QCU_PREC_DISPATCH(prec, {
         using CS = typename CxStore<typename PrecT<P>::S>::type;
         OmegaOp<CS> op{static_cast<const CS *>(t), static_cast<const CS *>(sv), st, tab.Rtot};
-        return run_seg_reduce(op, tab, w, 1, s);
+        return run_seg_reduce(op, tab, w, s);
     })
 }
 
@@ -703,7 +712,7 @@ hipError_t seg_step_xr(void *x, void *r, const void *p, const void *sv, const vo
         XROp<CS, T> op{static_cast<CS *>(x), static_cast<CS *>(r), static_cast<const CS *>(p),
                        static_cast<const CS *>(sv), static_cast<const CS *>(sv), static_cast<const CS *>(t),
                        static_cast<const CS *>(r0), st, tab.Rtot};
-        return run_seg_reduce<decltype(op), true>(op, tab, w, 2, s, st, done, iters);
+        return run_seg_reduce<decltype(op), true>(op, tab, w, s, st, done, iters);
     })
 }
 

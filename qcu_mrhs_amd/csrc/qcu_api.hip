@@ -369,8 +369,8 @@ int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditione
         set_error(kQcuErrorUnsupported, "inner precision %d", inner_precision);
         return g_err;
     }
-    if (!(delta > 0.0 && delta < 1.0)) {
-        set_error(kQcuErrorInvalidValue, "delta must be in (0, 1)");
+    if (!(delta >= 0.0 && delta < 1.0)) {
+        set_error(kQcuErrorInvalidValue, "delta must be in [0, 1) (0 = default for the inner precision)");
         return g_err;
     }
     c->opts.inner_prec = inner_precision;

@@ -21,7 +21,8 @@ public:
 
 struct SolverOptions {
     int inner_prec = kDouble;  // kDouble: plain FP64 BiCGStab; else reliable-update mixed precision
-    double delta = 0.1;        // reliable update when ||r|| < delta * max ||r|| since last update
+    double delta = 0.0;        // reliable update when ||r|| < delta * max ||r|| since last update
+                               // (0: 1e-3 FP32, 5e-3 FP16, 1e-2 BF16 inner precision)
     int use_precond = 0;       // apply the learned right preconditioner (inner solve only)
     double diverge = 1e6;      // stop (not converged) once a relative residual estimate exceeds this
     int verify_residual = 0;   // recompute b - M x on the full lattice after a mixed solve (else the

@@ -55,7 +55,7 @@ def test_start_dslash_and_mat_qcu_reference_protocol():
     api.finalizeQcu()
 
 
-def _solve(L, Nc, R, gauge, b, kappa, tol, inner=2, max_iter=3000, delta=0.1):
+def _solve(L, Nc, R, gauge, b, kappa, tol, inner=2, max_iter=3000, delta=0.0):
     _setup(L, Nc, R, gauge, kappa, inner=inner)
     api.set_solver_options(inner, delta, False)
     bs = [api.DeviceArray.from_complex(b[r]) for r in range(R)]
