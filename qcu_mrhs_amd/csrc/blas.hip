@@ -11,6 +11,8 @@
 // alpha = rho / <r0, v>) on device, so no host round trip is needed.  Results
 // are bit-reproducible for a given (n, R).  Accumulation is FP64 for every
 // storage precision.
+#include <type_traits>
+
 #include "blas.h"
 
 namespace qcu {
@@ -69,6 +71,9 @@ __global__ __launch_bounds__(256) void k_axpy(CS *__restrict__ res, const CS *__
 }
 
 // ------------------------------------------------------------- reduction
+template <class Op, class = void> struct HasSplit { static constexpr bool value = false; };
+template <class Op> struct HasSplit<Op, std::void_t<decltype(Op::kSplit)>> { static constexpr bool value = Op::kSplit; };
+
 // Two kernels per reduction: every block reduces its lanes per RHS through
 // LDS in a fixed order and writes R partials; a one-block kernel then sums the
 // partials in block order and runs the op's finisher.  The kernel boundary
@@ -89,8 +94,16 @@ __device__ __forceinline__ void reduce_partial(const Op &op, long N, int R, doub
         long e = g;
         // four entries per trip keep more loads in flight (same summation order as one per trip)
         for (; e + 3 * stride < N; e += 4 * stride) {
+            if constexpr (HasSplit<Op>::value) {
+                typename Op::V v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) op.elem(e + u * stride, loc, acc);
+                for (int u = 0; u < 4; ++u) v[u] = op.load(e + u * stride);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) op.apply(e + u * stride, v[u], loc, acc);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) op.elem(e + u * stride, loc, acc);
+            }
         }
         for (; e < N; e += stride) op.elem(e, loc, acc);
     }
@@ -254,22 +267,36 @@ struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
     int R;
     struct L { Cx<T> al, om; };
     __device__ L local(int rhs) const { return {tc<T>(st[kAlpha * R + rhs]), tc<T>(st[kOmega * R + rhs])}; }
-    __device__ void elem(long e, L l, double (&acc)[NA]) const {
-        const Cx<T> sv = load_cx<T>(s + e);
-        const Cx<T> shv = shat == s ? sv : load_cx<T>(shat + e);
-        const Cx<T> xv = load_cx<T>(x + e) + l.al * load_cx<T>(p + e) + l.om * shv;
+    // split form: every load of a trip is issued before the first store (the stores could
+    // alias the later loads as far as the compiler knows, which would serialise them)
+    static constexpr bool kSplit = true;
+    struct V { CS x, p, s, sh, t, r0; };
+    __device__ V load(long e) const {
+        V v;
+        v.s = s[e];
+        v.sh = shat == s ? v.s : shat[e];
+        v.x = x[e];
+        v.p = p[e];
+        v.t = t[e];
+        v.r0 = r0[e];
+        return v;
+    }
+    __device__ void apply(long e, const V &v, L l, double (&acc)[NA]) const {
+        const Cx<T> sv = load_cx<T>(&v.s);
+        const Cx<T> xv = load_cx<T>(&v.x) + l.al * load_cx<T>(&v.p) + l.om * load_cx<T>(&v.sh);
         store_cx(x + e, xv);
-        const Cx<T> rv = sv - l.om * load_cx<T>(t + e);
+        const Cx<T> rv = sv - l.om * load_cx<T>(&v.t);
         // reductions use the stored (rounded) residual: round in registers, not through memory
         CS rs;
         store_cx(&rs, rv);
         r[e] = rs;
         const Cx<double> rd = load_cx<double>(&rs);
-        const Cx<double> u = load_cx<double>(r0 + e);
+        const Cx<double> u = load_cx<double>(&v.r0);
         acc[0] = fma(rd.re, rd.re, fma(rd.im, rd.im, acc[0]));
         acc[1] = fma(u.re, rd.re, fma(u.im, rd.im, acc[1]));
         acc[2] = fma(u.re, rd.im, fma(-u.im, rd.re, acc[2]));
     }
+    __device__ void elem(long e, L l, double (&acc)[NA]) const { apply(e, load(e), l, acc); }
     __device__ void finish(int rr, const double (&tt)[NA]) const {
         st[kRnorm2 * R + rr] = {tt[0], 0.0};
         const C64 rho_new = {tt[1], tt[2]};
