@@ -42,6 +42,9 @@ int reduce_blocks(long n, int R, int max_blocks);
 // <a,b>_r -> res[r] (complex, FP64 accumulation)
 hipError_t launch_dotc(C64 *res, const void *a, const void *b, long n, int R, int prec,
                        ReduceWork &w, int slot, hipStream_t s);
+// res = x - y and ||res||_r -> nrm[r] in one pass (sqrt when take_sqrt)
+hipError_t launch_xsay_norm(double *nrm, void *res, const void *x, const void *y, long n, int R, int prec,
+                            bool take_sqrt, ReduceWork &w, int slot, hipStream_t s);
 // ||a||_r -> res[r] (sqrt applied when take_sqrt, else squared norm)
 hipError_t launch_norm(double *res, const void *a, long n, int R, int prec, bool take_sqrt,
                        ReduceWork &w, int slot, hipStream_t s);

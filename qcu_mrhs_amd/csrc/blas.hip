@@ -258,6 +258,34 @@ struct OmegaOp {  // <t,s>, <t,t>, omega = <t,s>/<t,t>
     __device__ OmegaOp shifted(long off, int base) const { return {t + off, s + off, st + base, R}; }
 };
 
+template <typename CS>
+struct XsayNormOp {  // res = x - y ; ||res|| per RHS (FP64 accumulation of the stored values)
+    static constexpr int NA = 1;
+    CS *res;
+    const CS *x, *y;
+    double *nrm;
+    bool take_sqrt;
+    static constexpr bool kSplit = true;
+    struct L {};
+    struct V { CS x, y; };
+    __device__ L local(int) const { return {}; }
+    __device__ V load(long e) const { return {x[e], y[e]}; }
+    __device__ void apply(long e, const V &v, L, double (&acc)[NA]) const {
+        using T = decltype(load_cx<double>(&v.x).re);
+        const Cx<T> d = load_cx<T>(&v.x) - load_cx<T>(&v.y);
+        CS rs;
+        store_cx(&rs, d);
+        res[e] = rs;
+        const Cx<double> u = load_cx<double>(&rs);
+        acc[0] = fma(u.re, u.re, fma(u.im, u.im, acc[0]));
+    }
+    __device__ void elem(long e, L l, double (&acc)[NA]) const { apply(e, load(e), l, acc); }
+    __device__ void finish(int r, const double (&t)[NA]) const { nrm[r] = take_sqrt ? sqrt(t[0]) : t[0]; }
+    __device__ XsayNormOp shifted(long off, int base) const {
+        return {res + off, x + off, y + off, nrm + base, take_sqrt};
+    }
+};
+
 template <typename CS, typename T>
 struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
     static constexpr int NA = 3;
@@ -591,6 +619,16 @@ hipError_t launch_convert(void *dst, int dst_prec, const void *src, int src_prec
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
+    })
+}
+
+hipError_t launch_xsay_norm(double *nrm, void *res, const void *x, const void *y, long n, int R, int prec,
+                            bool take_sqrt, ReduceWork &w, int slot, hipStream_t s) {
+    QCU_PREC_DISPATCH(prec, {
+        using CS = typename CxStore<typename PrecT<P>::S>::type;
+        XsayNormOp<CS> op{static_cast<CS *>(res), static_cast<const CS *>(x), static_cast<const CS *>(y), nrm,
+                          take_sqrt};
+        return run_reduce(op, n, R, w, slot, s);
     })
 }
 

@@ -25,7 +25,6 @@
 // columns beyond Nc read clamped, valid addresses and are never stored, so
 // the kernel has no divergent branches around its loads.
 #include <cstdint>
-#include <cstdlib>
 
 #include "dslash_common.h"
 #include "geom.h"
@@ -103,22 +102,13 @@ __device__ __forceinline__ uint32_t apply(const WordOp &o, uint32_t w) {
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), 0);
 }
-// non-temporal variants (cache policy NT): for data touched once per launch (links, the
-// output, the xpay input) so that it does not displace the reused neighbour spinors in L2
-constexpr int kCpolNT = 2;
-__device__ __forceinline__ uint32_t bload_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    return uint32_t(__builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), kCpolNT));
-}
-__device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, uint32_t v, unsigned voff, unsigned soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), kCpolNT);
-}
 
 // One wavefront = `spw` consecutive traversal positions x one 16-RHS chunk
 // (blockIdx.y).  Per site the wavefront needs one scalar row of the walk
 // table (site + 8 neighbours), prefetched one site ahead, so the only exposed
 // memory latency per site is that of its link and spinor loads, which are
 // all issued back to back before any of them is waited for.
-template <typename S, int NC, bool DAG, bool NT>
+template <typename S, int NC, bool DAG>
 __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ gauge,
                                                     const uint32_t *__restrict__ xpay, float coef, int Vh, int R,
@@ -235,10 +225,7 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
 #pragma unroll
         for (int st = 0; st < STEPS; ++st)
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                const unsigned ga = gaddr[st] + unsigned(cc) * gstep[st];
-                lw[st][cc] = cc < CPL ? (NT ? bload_nt(r_g, ga, 0u) : bload(r_g, ga, 0u)) : 0u;
-            }
+            for (int cc = 0; cc < 4; ++cc) lw[st][cc] = cc < CPL ? bload(r_g, gaddr[st] + unsigned(cc) * gstep[st], 0u) : 0u;
 
         f32x4 d[4];
 #pragma unroll
@@ -274,12 +261,11 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
                         const unsigned so = unsigned(sp * NC + i) * row_bytes;
                         float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
                         if (xpay) {
-                            const uint32_t xw = NT ? bload_nt(r_x, obase, so) : bload(r_x, obase, so);
+                            const uint32_t xw = bload(r_x, obase, so);
                             vr += P::re(xw);
                             vi += P::im(xw);
                         }
-                        if (NT) bstore_nt(r_out, P::pack(vr, vi), obase, so);
-                        else bstore(r_out, P::pack(vr, vi), obase, so);
+                        bstore(r_out, P::pack(vr, vi), obase, so);
                     }
                 }
             }
@@ -289,15 +275,6 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
             for (int k = 0; k < 9; ++k) cur[k] = nxt[k];
         }
     }
-}
-
-// QCU_DSLASH_NT=1 selects non-temporal loads/stores for the once-touched data (measurement knob)
-bool stream_hint() {
-    static const bool on = [] {
-        const char *e = std::getenv("QCU_DSLASH_NT");
-        return e && e[0] == '1';
-    }();
-    return on;
 }
 
 template <typename S, int NC>
@@ -311,18 +288,12 @@ hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) 
     auto xp = static_cast<const uint32_t *>(a.xpay);
     const unsigned vec_bytes = unsigned(size_t(a.g.Vh) * 4 * a.Nc * a.R * 4);
     const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
-#define QCU_MFMA(DAG, NT)                                                                                      \
-    hipLaunchKernelGGL((k_dslash_mfma<S, NC, DAG, NT>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef), \
-                       a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes)
-    const bool nt = stream_hint();
-    if (a.dagger) {
-        if (nt) QCU_MFMA(true, true);
-        else QCU_MFMA(true, false);
-    } else {
-        if (nt) QCU_MFMA(false, true);
-        else QCU_MFMA(false, false);
-    }
-#undef QCU_MFMA
+    if (a.dagger)
+        hipLaunchKernelGGL((k_dslash_mfma<S, NC, true>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
+                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
+    else
+        hipLaunchKernelGGL((k_dslash_mfma<S, NC, false>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
+                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
     return hipGetLastError();
 }
 

@@ -178,9 +178,7 @@ hipError_t BiCGStab::true_residual(const void *b_full, const void *x_full, const
     for (int p = 0; p < 2; ++p) {
         // tmp = x_p - kappa D_p x_{1-p} = (M x)_p ; v = b_p - tmp
         QCU_HIP_TRY(dslash(tmp64_, x + (1 - p) * nR, g64, kDouble, p, x + p * nR, -kappa, 0, s));
-        AxpyArgs ax{kXsay, v64_, b + p * nR, tmp64_, nullptr, ones_, nullptr, nullptr, n_, R_, kDouble};
-        QCU_HIP_TRY(launch_axpy(ax, s));
-        QCU_HIP_TRY(launch_norm(dn_, v64_, n_, R_, kDouble, false, red_, 4, s));
+        QCU_HIP_TRY(launch_xsay_norm(dn_, v64_, b + p * nR, tmp64_, n_, R_, kDouble, false, red_, 4, s));
         QCU_HIP_TRY(launch_norm(dn_ + R_, b + p * nR, n_, R_, kDouble, false, red_, 5, s));
         QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * 2 * R_, hipMemcpyDeviceToHost, s));
         QCU_HIP_TRY(hipStreamSynchronize(s));
@@ -329,9 +327,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                 QCU_HIP_TRY(hipGetLastError());
             }
             QCU_HIP_TRY(apply_schur(v64_, x_o, tmp64_, gauge.g64, kDouble, kappa, 0, s));
-            AxpyArgs ax{kXsay, rhi_, bp_, v64_, nullptr, ones_, nullptr, nullptr, n_, R_, kDouble};
-            QCU_HIP_TRY(launch_axpy(ax, s));
-            QCU_HIP_TRY(launch_norm(dn_, rhi_, n_, R_, kDouble, true, red_, 7, s));
+            QCU_HIP_TRY(launch_xsay_norm(dn_, rhi_, bp_, v64_, n_, R_, kDouble, true, red_, 7, s));
             QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
             QCU_HIP_TRY(hipStreamSynchronize(s));
             res.reliable_updates++;
