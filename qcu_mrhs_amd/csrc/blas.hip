@@ -295,9 +295,11 @@ struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
     int R;
     struct L { Cx<T> al, om; };
     __device__ L local(int rhs) const { return {tc<T>(st[kAlpha * R + rhs]), tc<T>(st[kOmega * R + rhs])}; }
-    // split form: every load of a trip is issued before the first store (the stores could
-    // alias the later loads as far as the compiler knows, which would serialise them)
-    static constexpr bool kSplit = true;
+    // split form for 16-bit and 32-bit storage: every load of a trip is issued before the first
+    // store (the stores could alias the later loads as far as the compiler knows, which would
+    // serialise them); FP64 keeps one entry at a time (24 in-flight 16-byte loads per lane cost
+    // more occupancy than they gain)
+    static constexpr bool kSplit = sizeof(CS) <= 8;
     struct V { CS x, p, s, sh, t, r0; };
     __device__ V load(long e) const {
         V v;
