@@ -45,6 +45,10 @@ hipError_t launch_dotc(C64 *res, const void *a, const void *b, long n, int R, in
 // res = x - y and ||res||_r -> nrm[r] in one pass (sqrt when take_sqrt)
 hipError_t launch_xsay_norm(double *nrm, void *res, const void *x, const void *y, long n, int R, int prec,
                             bool take_sqrt, ReduceWork &w, int slot, hipStream_t s);
+// FP64 residual d = b - y, ||d||_r -> nrm[r] (sqrt), and lo = scale[r] d in the
+// inner storage `lo_prec` (FP32/FP16/BF16), in one pass; d itself is not stored
+hipError_t launch_residual_lo(double *nrm, void *lo, int lo_prec, const void *b, const void *y, const double *scale,
+                              long n, int R, ReduceWork &w, int slot, hipStream_t s);
 // ||a||_r -> res[r] (sqrt applied when take_sqrt, else squared norm)
 hipError_t launch_norm(double *res, const void *a, long n, int R, int prec, bool take_sqrt,
                        ReduceWork &w, int slot, hipStream_t s);

@@ -286,6 +286,30 @@ struct XsayNormOp {  // res = x - y ; ||res|| per RHS (FP64 accumulation of the 
     }
 };
 
+template <typename CL>
+struct ResidualLoOp {  // d = b - y in FP64 ; ||d|| per RHS ; lo = scale_r d rounded to the inner storage
+    static constexpr int NA = 1;
+    static constexpr bool kSplit = true;
+    CL *lo;
+    const Cx<double> *b, *y;
+    const double *scale;
+    double *nrm;
+    struct L { double sc; };
+    struct V { Cx<double> b, y; };
+    __device__ L local(int rhs) const { return {scale[rhs]}; }
+    __device__ V load(long e) const { return {b[e], y[e]}; }
+    __device__ void apply(long e, const V &v, L l, double (&acc)[NA]) const {
+        const Cx<double> d = v.b - v.y;
+        acc[0] = fma(d.re, d.re, fma(d.im, d.im, acc[0]));
+        store_cx(lo + e, l.sc * d);
+    }
+    __device__ void elem(long e, L l, double (&acc)[NA]) const { apply(e, load(e), l, acc); }
+    __device__ void finish(int r, const double (&t)[NA]) const { nrm[r] = sqrt(t[0]); }
+    __device__ ResidualLoOp shifted(long off, int base) const {
+        return {lo + off, b + off, y + off, scale + base, nrm + base};
+    }
+};
+
 template <typename CS, typename T>
 struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
     static constexpr int NA = 3;
@@ -622,6 +646,18 @@ hipError_t launch_convert(void *dst, int dst_prec, const void *src, int src_prec
         }
         return hipGetLastError();
     })
+}
+
+hipError_t launch_residual_lo(double *nrm, void *lo, int lo_prec, const void *b, const void *y, const double *scale,
+                              long n, int R, ReduceWork &w, int slot, hipStream_t s) {
+    const auto *bb = static_cast<const Cx<double> *>(b);
+    const auto *yy = static_cast<const Cx<double> *>(y);
+    switch (lo_prec) {
+        case kSingle: return run_reduce(ResidualLoOp<Cx<float>>{static_cast<Cx<float> *>(lo), bb, yy, scale, nrm}, n, R, w, slot, s);
+        case kHalf: return run_reduce(ResidualLoOp<Half2Store>{static_cast<Half2Store *>(lo), bb, yy, scale, nrm}, n, R, w, slot, s);
+        case kBF16: return run_reduce(ResidualLoOp<Bf162Store>{static_cast<Bf162Store *>(lo), bb, yy, scale, nrm}, n, R, w, slot, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_xsay_norm(double *nrm, void *res, const void *x, const void *y, long n, int R, int prec,

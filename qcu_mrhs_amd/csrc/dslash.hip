@@ -209,9 +209,10 @@ DslashTune dslash_resolve(const DslashArgs &a) {
     if (t.variant != kDslashLaneRhs) {
         if (t.spw <= 0) t.spw = t.variant == kDslashMfma ? 1 : 4;  // measured best at 24^3x48, R = 16
         // tile the (y, z) plane so one tile slice (the t-reuse distance) stays
-        // around 1 MiB: bytes per site = 4 Nc R complex numbers
+        // around 256 KiB: bytes per site = 4 Nc R complex numbers (measured at
+        // 24^3x48, 16 RHS: best among tiles of 0.06 - 5 MiB for FP16 and FP64)
         const double site_bytes = 4.0 * a.Nc * a.R * 2.0 * double(prec_bytes(a.prec));
-        double area = (1 << 20) / (site_bytes * a.g.Lxh);
+        double area = (1 << 18) / (site_bytes * a.g.Lxh);
         if (area < 1) area = 1;
         if (t.by <= 0 || a.g.L[1] % t.by) {
             int cap = 1;

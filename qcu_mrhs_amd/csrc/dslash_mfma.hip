@@ -107,8 +107,10 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, uns
 // (blockIdx.y).  Per site the wavefront needs one scalar row of the walk
 // table (site + 8 neighbours), prefetched one site ahead, so the only exposed
 // memory latency per site is that of its link and spinor loads, which are
-// all issued back to back before any of them is waited for.
-template <typename S, int NC, bool DAG>
+// all issued back to back before any of them is waited for.  (Computing the
+// row from the traversal index instead costs about 500 scalar instructions
+// per site, which would make the scalar unit the bottleneck.)
+template <typename S, int NC, bool DAG, bool XP>
 __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ gauge,
                                                     const uint32_t *__restrict__ xpay, float coef, int Vh, int R,
@@ -226,6 +228,15 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         for (int st = 0; st < STEPS; ++st)
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) lw[st][cc] = cc < CPL ? bload(r_g, gaddr[st] + unsigned(cc) * gstep[st], 0u) : 0u;
+        // the xpay words of this lane's output rows are needed only in the epilogue: issue them
+        // now so their latency overlaps the MFMA work instead of serialising every store
+        const unsigned xbase = unsigned(site) * site_bytes + rcb + unsigned(2 * grp) * row_bytes;
+        uint32_t xw[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp)
+                xw[i][sp] = (XP && 2 * grp + i < NC) ? bload(r_x, xbase, unsigned(sp * NC + i) * row_bytes) : 0u;
 
         f32x4 d[4];
 #pragma unroll
@@ -260,10 +271,9 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
                     for (int sp = 0; sp < 4; ++sp) {
                         const unsigned so = unsigned(sp * NC + i) * row_bytes;
                         float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
-                        if (xpay) {
-                            const uint32_t xw = bload(r_x, obase, so);
-                            vr += P::re(xw);
-                            vi += P::im(xw);
+                        if constexpr (XP) {
+                            vr += P::re(xw[i][sp]);
+                            vi += P::im(xw[i][sp]);
                         }
                         bstore(r_out, P::pack(vr, vi), obase, so);
                     }
@@ -288,12 +298,11 @@ hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) 
     auto xp = static_cast<const uint32_t *>(a.xpay);
     const unsigned vec_bytes = unsigned(size_t(a.g.Vh) * 4 * a.Nc * a.R * 4);
     const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
-    if (a.dagger)
-        hipLaunchKernelGGL((k_dslash_mfma<S, NC, true>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
-                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
-    else
-        hipLaunchKernelGGL((k_dslash_mfma<S, NC, false>), grid, block, 0, stream, out, in, gauge, xp, float(a.coef),
-                           a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes);
+    // xpay is a separate instance: its prefetched words cost registers the plain form keeps
+    auto k = a.dagger ? (xp ? k_dslash_mfma<S, NC, true, true> : k_dslash_mfma<S, NC, true, false>)
+                      : (xp ? k_dslash_mfma<S, NC, false, true> : k_dslash_mfma<S, NC, false, false>);
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g.Vh, a.R, a.parity,
+                       tab.walk, t.spw, vec_bytes, gauge_bytes);
     return hipGetLastError();
 }
 

@@ -89,12 +89,12 @@ private:
     void *r0_ = nullptr, *r_ = nullptr, *p_ = nullptr, *v_ = nullptr, *s_ = nullptr, *t_ = nullptr,
          *tmp_ = nullptr, *xlo_ = nullptr, *ph_ = nullptr, *sh_ = nullptr;
     // FP64 vectors
-    void *bp_ = nullptr, *xhi_ = nullptr, *rhi_ = nullptr, *tmp64_ = nullptr, *v64_ = nullptr;
+    void *bp_ = nullptr, *tmp64_ = nullptr, *v64_ = nullptr;
     C64 *st_ = nullptr;        // scalar slots [kNumSlots][R]
     C64 *ones_ = nullptr;      // [R] of 1
     C64 *kap_ = nullptr;       // [R] of kappa
     double *dn_ = nullptr;     // [4][R] norms
-    double *dscale_ = nullptr; // [R] per-RHS scale
+    double *dscale_ = nullptr; // [2][R] per-RHS scales (accumulation, new inner residual)
     double *h_ = nullptr;      // pinned host staging [8][R]
     ReduceWork red_;
 };

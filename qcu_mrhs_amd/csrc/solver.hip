@@ -97,8 +97,7 @@ void dfree(void *&p) {
 BiCGStab::BiCGStab(const Geom &g, int Nc, int R) : g_(g), Nc_(Nc), R_(R), n_(long(g.Vh) * 4 * Nc) {}
 
 BiCGStab::~BiCGStab() {
-    for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &xhi_, &rhi_,
-                     &tmp64_, &v64_})
+    for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &tmp64_, &v64_})
         dfree(*q);
     void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_;
     dfree(a); dfree(b); dfree(c); dfree(d); dfree(e);
@@ -114,7 +113,7 @@ hipError_t BiCGStab::ensure(int prec) {
         ones_ = static_cast<C64 *>(dmalloc(sizeof(C64) * R_, err));
         kap_ = static_cast<C64 *>(dmalloc(sizeof(C64) * R_, err));
         dn_ = static_cast<double *>(dmalloc(sizeof(double) * 4 * R_, err));
-        dscale_ = static_cast<double *>(dmalloc(sizeof(double) * R_, err));
+        dscale_ = static_cast<double *>(dmalloc(sizeof(double) * 2 * R_, err));
         bp_ = dmalloc(half64, err);
         tmp64_ = dmalloc(half64, err);
         v64_ = dmalloc(half64, err);
@@ -125,7 +124,7 @@ hipError_t BiCGStab::ensure(int prec) {
         QCU_HIP_TRY(hipDeviceSynchronize());  // the solver runs on a non-blocking stream
     }
     if (prec >= 0 && alloc_prec_ != prec) {
-        for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &rhi_}) dfree(*q);
+        for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_}) dfree(*q);
         const size_t half = size_t(n_) * R_ * 2 * prec_bytes(prec);
         r0_ = dmalloc(half, err);
         r_ = dmalloc(half, err);
@@ -134,10 +133,7 @@ hipError_t BiCGStab::ensure(int prec) {
         s_ = dmalloc(half, err);
         t_ = dmalloc(half, err);
         tmp_ = dmalloc(half, err);
-        if (prec != kDouble) {
-            xlo_ = dmalloc(half, err);
-            rhi_ = dmalloc(half64, err);
-        }
+        if (prec != kDouble) xlo_ = dmalloc(half, err);
         if (err != hipSuccess) return err;
         alloc_prec_ = prec;
     }
@@ -313,9 +309,17 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
             continue;
         }
         if (conv_est || want_update) {
-            // ---- reliable update: fold x_lo into x_hi, recompute the FP64 true residual
-            for (int r = 0; r < R_; ++r) h_[r] = 1.0 / sigma[r];
-            QCU_HIP_TRY(hipMemcpyAsync(dscale_, h_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
+            // ---- reliable update: fold x_lo into x_hi, recompute the FP64 true residual.
+            // The new inner residual is written in the same pass, scaled by the inverse of
+            // the iterated residual norm (the true norm is only known after the pass; the
+            // two agree closely, and the pass is repeated with the true norm if not).
+            std::vector<double> sest(R_);
+            for (int r = 0; r < R_; ++r) {
+                sest[r] = rn[r] > 0 ? 1.0 / rn[r] : sigma[r];
+                h_[r] = 1.0 / sigma[r];
+                h_[R_ + r] = sest[r];
+            }
+            QCU_HIP_TRY(hipMemcpyAsync(dscale_, h_, sizeof(double) * 2 * R_, hipMemcpyHostToDevice, s));
             {
                 const int nb_ = div_up(nR, 256) > 8192 ? 8192 : div_up(nR, 256);
                 switch (P) {
@@ -327,7 +331,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                 QCU_HIP_TRY(hipGetLastError());
             }
             QCU_HIP_TRY(apply_schur(v64_, x_o, tmp64_, gauge.g64, kDouble, kappa, 0, s));
-            QCU_HIP_TRY(launch_xsay_norm(dn_, rhi_, bp_, v64_, n_, R_, kDouble, true, red_, 7, s));
+            QCU_HIP_TRY(launch_residual_lo(dn_, r_, P, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s));
             QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
             QCU_HIP_TRY(hipStreamSynchronize(s));
             res.reliable_updates++;
@@ -344,19 +348,30 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                 last_rt = rt;
                 break;
             }
-            // rescale: inner vectors become sigma' * (true quantities)
+            // rescale: inner vectors become sigma' * (true quantities), sigma' = sest unless
+            // sest * ||r_true|| left [1/16, 16] (then the pass is redone with 1 / ||r_true||)
+            bool redo = false;
             for (int r = 0; r < R_; ++r) {
-                const double sn = rt[r] > 0 ? 1.0 / rt[r] : sigma[r];
+                const double q = sest[r] * rt[r];
+                if (rt[r] > 0 && !(q >= 0.0625 && q <= 16.0)) {
+                    sest[r] = 1.0 / rt[r];
+                    redo = true;
+                }
+            }
+            for (int r = 0; r < R_; ++r) {
+                const double sn = sest[r];
                 h_[r] = sn;                             // sigma'
                 h_[R_ + 2 * r] = sn / sigma[r];         // c as complex {c, 0}
                 h_[R_ + 2 * r + 1] = 0.0;
                 sigma[r] = sn;
                 maxr[r] = rt[r];
             }
-            QCU_HIP_TRY(hipMemcpyAsync(dscale_, h_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
             QCU_HIP_TRY(hipMemcpyAsync(st_ + kPScale * R_, h_ + R_, sizeof(C64) * R_,
                                        hipMemcpyHostToDevice, s));
-            QCU_HIP_TRY(launch_convert(r_, P, rhi_, kDouble, nR, dscale_, R_, s));
+            if (redo) {
+                QCU_HIP_TRY(hipMemcpyAsync(dscale_ + R_, h_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
+                QCU_HIP_TRY(launch_residual_lo(dn_, r_, P, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s));
+            }
             QCU_HIP_TRY(hipMemsetAsync(xlo_, 0, bytes_lo, s));
             QCU_HIP_TRY(launch_dotc(st_ + kTmpB * R_, r0_, r_, n_, R_, P, red_, 8, s));
             hipLaunchKernelGGL(k_reliable_scalars, dim3(1), dim3((R_ + 63) / 64 * 64), 0, s, st_, R_);
