@@ -86,9 +86,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dslash_24c3x48_r16_hbm_traffic.md
 
 
 def measured_traffic(workload, prec_name, variant):
-    """HBM bytes per dslash launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (copy-kernel calibrated), for the workload and the
-    automatic kernel choice it was measured on; None otherwise."""
+    """HBM bytes per dslash launch from the committed rocprofv3 summary (read
+    bytes from the TCC read-request size counters, write bytes from WRITE_SIZE,
+    both checked against a copy kernel), for the workload and the automatic
+    kernel choice it was measured on; None otherwise."""
     if workload != "c3" or variant != "auto" or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):

@@ -106,6 +106,45 @@ def test_invert_small_mixed_precision(inner):
     assert max(err) < 1e-8
 
 
+@pytest.mark.parametrize("Nc,R,inner", [(2, 17, 0), (4, 17, 4), (3, 33, 0)])
+def test_invert_mixed_precision_rhs_chunks(Nc, R, inner):
+    """Matrix-core inner operator over more than one 16-RHS chunk (R = 17, 33:
+    the last chunk partly empty) and colour counts 2 / 4: FP64 true residual
+    <= 1e-10 through the oracle operator, solution within 1e-8 of the oracle."""
+    L, kappa, tol = (8, 8, 8, 8), 0.12, 1e-10
+    _case("invert mixed chunks", Nc, R, inner)
+    gauge = fields.gauge_field(L, Nc, 1100 + Nc, kind="haar")
+    b = fields.gaussian_sources(L, Nc, R, 1200 + R)
+    x, info = _solve(L, Nc, R, gauge, b, kappa, tol, inner=inner)
+    bm, xm = fields.to_mrhs(b, Nc), fields.to_mrhs(x, Nc)
+    r = bm - oracle.mat(xm, gauge, L, Nc, R, kappa)
+    true_rel = oracle.norm(r, R) / oracle.norm(bm, R)
+    xo, ok, it, _ = oracle.bicgstab(bm, gauge, L, Nc, R, kappa, 3000, tol)
+    print(f"Nc {Nc} R {R} inner {inner}: {info['iterations']} iterations (oracle FP64 {it}), "
+          f"{info['reliable_updates']} reliable updates, true residual {true_rel.max():.3e}, "
+          f"error vs oracle {_rel(xm, xo):.2e}")
+    assert info["converged"] and ok
+    assert true_rel.max() < 1.5e-10 and info["true_rel_residual"] < 1.5e-10
+    assert _rel(xm, xo) < 1e-8
+
+
+def test_invert_production_shape_mixed_half():
+    """16^3x32, R=12, exp(i 0.3 H), kappa=0.135 with FP16 inner vectors and
+    the matrix-core operator: FP64 true residual <= 1e-10 through the oracle."""
+    L, Nc, R, kappa, tol = (16, 16, 16, 32), 3, 12, 0.135, 1e-10
+    _case("invert production mixed half", L, R)
+    gauge = fields.gauge_field(L, Nc, 2024, kind="smooth", eps=0.3)
+    b = fields.gaussian_sources(L, Nc, R, 2026)
+    x, info = _solve(L, Nc, R, gauge, b, kappa, tol, inner=0, max_iter=20000)
+    bm = fields.to_mrhs(b, Nc)
+    r = bm - oracle.mat(fields.to_mrhs(x, Nc), gauge, L, Nc, R, kappa)
+    true_rel = oracle.norm(r, R) / oracle.norm(bm, R)
+    print(f"production mixed FP16: {info['iterations']} iterations, {info['reliable_updates']} reliable "
+          f"updates, true residual {true_rel.max():.3e} (library {info['true_rel_residual']:.3e}), "
+          f"{info['seconds']:.3f} s")
+    assert info["converged"] and true_rel.max() < 1.5e-10 and info["true_rel_residual"] < 1.5e-10
+
+
 def test_invert_production_shape_plain_fp64():
     """16^3x32, R=12, exp(i 0.3 H), kappa=0.135: converges to a FP64 true
     residual <= 1e-10 (checked independently through the oracle operator)."""
