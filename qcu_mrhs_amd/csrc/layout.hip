@@ -8,6 +8,8 @@
 // with precision conversion.  One lane per destination complex number, so
 // the RHS-innermost side is written with unit stride; the R source pointers
 // travel in the kernel arguments (no pointer table in device memory).
+#include <algorithm>
+
 #include "layout.h"
 
 namespace qcu {
@@ -25,15 +27,31 @@ __global__ __launch_bounds__(256) void k_gather(CD *__restrict__ dst, PtrTable s
     }
 }
 
+// Scatter through LDS: a block reads a tile of tj consecutive entries x all R
+// RHS contiguously from the RHS-innermost source, then writes tj consecutive
+// entries of each destination vector (one contiguous run per RHS).  Writing
+// straight from the source order would give every destination vector 64-byte
+// pieces per wavefront, with the halves of one 128-byte line written by
+// different blocks (measured 3.4 TB/s against 4.9 TB/s for the gather).
+// The LDS row pitch R + 1 keeps the transposed reads free of bank conflicts.
 template <typename CD, typename CS>
-__global__ __launch_bounds__(256) void k_scatter(PtrTable dst, const CS *__restrict__ src, long n, int R) {
-    const long N = n * R;
-    const long stride = long(gridDim.x) * blockDim.x;
-    for (long e = long(blockIdx.x) * blockDim.x + threadIdx.x; e < N; e += stride) {
-        const long j = e / R;
-        const int r = int(e - j * R);
-        CD *d = static_cast<CD *>(const_cast<void *>(dst.p[r]));
-        store_cx(d + j, load_cx<double>(src + e));
+__global__ __launch_bounds__(256) void k_scatter(PtrTable dst, const CS *__restrict__ src, long n, int R, int tj) {
+    extern __shared__ Cx<double> tile[];  // [tj][R + 1]
+    const int pitch = R + 1;
+    for (long j0 = long(blockIdx.x) * tj; j0 < n; j0 += long(gridDim.x) * tj) {
+        const int cnt = int(n - j0 < tj ? n - j0 : tj);
+        const long base = j0 * R;
+        for (int i = threadIdx.x; i < cnt * R; i += blockDim.x) {
+            const int jj = i / R, r = i - jj * R;
+            tile[jj * pitch + r] = load_cx<double>(src + base + i);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt * R; i += blockDim.x) {
+            const int r = i / cnt, jj = i - r * cnt;
+            CD *d = static_cast<CD *>(const_cast<void *>(dst.p[r]));
+            store_cx(d + j0 + jj, tile[jj * pitch + r]);
+        }
+        __syncthreads();
     }
 }
 
@@ -84,12 +102,17 @@ hipError_t gather_src(void *dst, const PtrTable &src, int src_prec, long n, int 
 
 template <typename CS>
 hipError_t scatter_dst(const PtrTable &dst, int dst_prec, const void *src, long n, int R, hipStream_t s) {
-    const int nb = grid_for(n * R);
+    // tile of about 2176 complex numbers (34 KiB of LDS with the padded pitch)
+    const int tj = std::max(1, 2176 / (R + 1));
+    const size_t lds = size_t(tj) * (R + 1) * sizeof(Cx<double>);
+    const long tiles = (n + tj - 1) / tj;
+    const int nb = int(tiles < 8192 ? (tiles > 0 ? tiles : 1) : 8192);
+    const auto *sv = static_cast<const CS *>(src);
     switch (dst_prec) {
-        case kDouble: hipLaunchKernelGGL((k_scatter<Cx<double>, CS>), dim3(nb), dim3(256), 0, s, dst, static_cast<const CS *>(src), n, R); break;
-        case kSingle: hipLaunchKernelGGL((k_scatter<Cx<float>, CS>), dim3(nb), dim3(256), 0, s, dst, static_cast<const CS *>(src), n, R); break;
-        case kHalf: hipLaunchKernelGGL((k_scatter<Half2Store, CS>), dim3(nb), dim3(256), 0, s, dst, static_cast<const CS *>(src), n, R); break;
-        case kBF16: hipLaunchKernelGGL((k_scatter<Bf162Store, CS>), dim3(nb), dim3(256), 0, s, dst, static_cast<const CS *>(src), n, R); break;
+        case kDouble: hipLaunchKernelGGL((k_scatter<Cx<double>, CS>), dim3(nb), dim3(256), lds, s, dst, sv, n, R, tj); break;
+        case kSingle: hipLaunchKernelGGL((k_scatter<Cx<float>, CS>), dim3(nb), dim3(256), lds, s, dst, sv, n, R, tj); break;
+        case kHalf: hipLaunchKernelGGL((k_scatter<Half2Store, CS>), dim3(nb), dim3(256), lds, s, dst, sv, n, R, tj); break;
+        case kBF16: hipLaunchKernelGGL((k_scatter<Bf162Store, CS>), dim3(nb), dim3(256), lds, s, dst, sv, n, R, tj); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
