@@ -399,8 +399,16 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
         QCU_HIP_TRY(hipStreamSynchronize(s));
     }
     res.converged = converged ? 1 : 0;
-    // x_e = b_e + kappa D_eo x_o
-    QCU_HIP_TRY(dslash(x_e, x_o, gauge.g64, kDouble, 0, b_e, kappa, 0, s));
+    // x_e = b_e + kappa D_eo x_o.  A mixed solve that ended on a reliable update has
+    // D_eo x_o of the final x_o in tmp64_ already (first dslash of that update's FP64
+    // Schur operator): an axpy replaces the dslash.
+    if (!last_rt.empty()) {
+        QCU_HIP_TRY(launch_scalar_fill(kap_, C64{kappa, 0.0}, R_, s));
+        AxpyArgs ax{kXpay, x_e, b_e, tmp64_, nullptr, kap_, nullptr, nullptr, n_, R_, kDouble};
+        QCU_HIP_TRY(launch_axpy(ax, s));
+    } else {
+        QCU_HIP_TRY(dslash(x_e, x_o, gauge.g64, kDouble, 0, b_e, kappa, 0, s));
+    }
     double mx = 0.0;
     if (!last_rt.empty() && !opt.verify_residual) {
         // With x_e = b_e + kappa D_eo x_o the even rows of b - M x vanish and the odd rows equal
