@@ -73,6 +73,11 @@ __global__ __launch_bounds__(256) void k_axpy(CS *__restrict__ res, const CS *__
 // ------------------------------------------------------------- reduction
 template <class Op, class = void> struct HasSplit { static constexpr bool value = false; };
 template <class Op> struct HasSplit<Op, std::void_t<decltype(Op::kSplit)>> { static constexpr bool value = Op::kSplit; };
+// entries per trip of the split form (default 4; an op with wide entries may ask for fewer)
+template <class Op, class = void> struct SplitWidth { static constexpr int value = 4; };
+template <class Op> struct SplitWidth<Op, std::void_t<decltype(Op::kSplitWidth)>> {
+    static constexpr int value = Op::kSplitWidth;
+};
 
 // Two kernels per reduction: every block reduces its lanes per RHS through
 // LDS in a fixed order and writes R partials; a one-block kernel then sums the
@@ -92,15 +97,18 @@ __device__ __forceinline__ void reduce_partial(const Op &op, long N, int R, doub
     if (rhs_lane(R, g, stride, rhs)) {
         auto loc = op.local(rhs);
         long e = g;
-        // four entries per trip keep more loads in flight (same summation order as one per trip)
-        for (; e + 3 * stride < N; e += 4 * stride) {
-            if constexpr (HasSplit<Op>::value) {
-                typename Op::V v[4];
+        // several entries per trip keep more loads in flight (same summation order as one per trip)
+        if constexpr (HasSplit<Op>::value) {
+            constexpr int W = SplitWidth<Op>::value;
+            for (; e + (W - 1) * stride < N; e += W * stride) {
+                typename Op::V v[W];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) v[u] = op.load(e + u * stride);
+                for (int u = 0; u < W; ++u) v[u] = op.load(e + u * stride);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) op.apply(e + u * stride, v[u], loc, acc);
-            } else {
+                for (int u = 0; u < W; ++u) op.apply(e + u * stride, v[u], loc, acc);
+            }
+        } else {
+            for (; e + 3 * stride < N; e += 4 * stride) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) op.elem(e + u * stride, loc, acc);
             }
@@ -319,11 +327,12 @@ struct XROp {  // x += a p + w shat ; r = s - w t ; ||r||^2 ; <r0, r> ; beta
     int R;
     struct L { Cx<T> al, om; };
     __device__ L local(int rhs) const { return {tc<T>(st[kAlpha * R + rhs]), tc<T>(st[kOmega * R + rhs])}; }
-    // split form for 16-bit and 32-bit storage: every load of a trip is issued before the first
-    // store (the stores could alias the later loads as far as the compiler knows, which would
-    // serialise them); FP64 keeps one entry at a time (24 in-flight 16-byte loads per lane cost
-    // more occupancy than they gain)
-    static constexpr bool kSplit = sizeof(CS) <= 8;
+    // split form: every load of a trip is issued before the first store (the stores could
+    // alias the later loads as far as the compiler knows, which would serialise them); FP64
+    // entries take two per trip (four would hold 20 16-byte loads per lane, which costs more
+    // occupancy than it gains)
+    static constexpr bool kSplit = true;
+    static constexpr int kSplitWidth = sizeof(CS) <= 8 ? 4 : 2;
     struct V { CS x, p, s, sh, t, r0; };
     __device__ V load(long e) const {
         V v;
@@ -578,6 +587,8 @@ int reduce_blocks(long n, int R, int max_blocks) {
 }
 
 hipError_t reduce_work_alloc(ReduceWork &w, int R) {
+    // 1024 blocks (4 per CU); 2048 measured slower for the 16-bit reductions and no faster
+    // for the FP64 ones
     w.max_blocks = 1024;
     // one partials region per slot: reductions on different slots may be in flight together
     QCU_HIP_TRY(hipMalloc(&w.partials, sizeof(double) * 4 * size_t(w.max_blocks) * R * kReduceSlots));
