@@ -68,7 +68,8 @@ __device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS
                                            T coef, const Geom &g, int R, int parity, int by, int bz,
                                            int spw) {
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    // wavefront-uniform: the site's coordinates and neighbours go to the scalar unit
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mu = lane >> 4;
     const int rl = lane & 15;
     const long site_len = 4L * NC * R;
