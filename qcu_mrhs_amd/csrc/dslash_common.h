@@ -19,18 +19,6 @@ __host__ __device__ constexpr int kReconK(int mu, int r) {
     return mu == 0 ? 3 : mu == 1 ? (r == 0 ? 2 : 0) : mu == 2 ? (r == 0 ? 3 : 1) : 0;
 }
 
-// z * i^k for a run-time k in 0..3 (selects and sign flips only)
-template <typename T>
-__device__ __forceinline__ Cx<T> ipow_rt(Cx<T> z, int k) {
-    T re = (k & 1) ? -z.im : z.re;
-    T im = (k & 1) ? z.re : z.im;
-    if (k & 2) {
-        re = -re;
-        im = -im;
-    }
-    return {re, im};
-}
-
 // Constants of one hop, selected at run time per lane.
 struct HopInfo {
     int kp0, kp1;   // projection factors i^kp for rows 0 and 1

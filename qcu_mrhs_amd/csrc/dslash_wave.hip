@@ -22,20 +22,60 @@ __device__ __forceinline__ Cx<T> shfl_xor_cx(Cx<T> v, int mask) {
     return {__shfl_xor(v.re, mask), __shfl_xor(v.im, mask)};
 }
 
+// Per-lane constants of the hops of one direction in arithmetic form:
+// z * i^k = (c re - s im, s re + c im) with (c, s) = (1, 0), (0, 1), (-1, 0),
+// (0, -1) for k = 0..3, and the projection's partner rows as element offsets.
+// Multiplying by (c, s) costs four FMAs where a run-time select of the 16-byte
+// FP64 components costs about ten VALU operations.  The constants are those of
+// the forward hop; every factor of the backward hop is the negative (its sign
+// exponent differs by 2), applied as an FMA operand modifier.
+template <typename T>
+struct HopMul {
+    T cp0, sp0, cp1, sp1;  // projection factors of rows 0 and 1
+    T cr0, sr0, cr1, sr1;  // reconstruction factors
+    int ob0, ob1;          // element offsets of the partner rows of rows 0 and 1
+};
+
+template <typename T>
+__device__ __forceinline__ void ipow_cs(int k, T &c, T &s) {
+    c = (k & 1) ? T(0) : ((k & 2) ? T(-1) : T(1));
+    s = (k & 1) ? ((k & 2) ? T(-1) : T(1)) : T(0);
+}
+
+template <typename T, int NC>
+__device__ __forceinline__ HopMul<T> hop_mul(const HopInfo &h, int R) {
+    HopMul<T> m;
+    ipow_cs(h.kp0, m.cp0, m.sp0);
+    ipow_cs(h.kp1, m.cp1, m.sp1);
+    ipow_cs(h.kr0, m.cr0, m.sr0);
+    ipow_cs(h.kr1, m.cr1, m.sr1);
+    m.ob0 = (h.m0_is_2 ? 2 : 3) * NC * R;
+    m.ob1 = (h.m0_is_2 ? 3 : 2) * NC * R;
+    return m;
+}
+
+// a + (c, s) * b
+template <typename T>
+__device__ __forceinline__ Cx<T> add_rot(Cx<T> a, Cx<T> b, T c, T s) {
+    return {fma(c, b.re, fma(-s, b.im, a.re)), fma(s, b.re, fma(c, b.im, a.im))};
+}
+
+// One hop of this lane's direction.  acc[0], acc[1] collect the upper spins,
+// acc[2], acc[3] the reconstructed rows in projection order (partner of row 0,
+// partner of row 1); the caller maps them to spins 2 and 3 once per site.
 template <typename T, typename CS, int NC, bool FWD>
 __device__ __forceinline__ void hop_rt(Cx<T> (&acc)[4][NC], const CS *__restrict__ psi, int R,
-                                       const CS *__restrict__ U, const HopInfo &h) {
+                                       const CS *__restrict__ U, const HopMul<T> &h) {
     Cx<T> h0[NC], h1[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const Cx<T> a0 = load_cx<T>(psi + (0 * NC + c) * R);
         const Cx<T> a1 = load_cx<T>(psi + (1 * NC + c) * R);
-        const Cx<T> a2 = load_cx<T>(psi + (2 * NC + c) * R);
-        const Cx<T> a3 = load_cx<T>(psi + (3 * NC + c) * R);
-        const Cx<T> b0 = h.m0_is_2 ? a2 : a3;
-        const Cx<T> b1 = h.m0_is_2 ? a3 : a2;
-        h0[c] = a0 + ipow_rt(b0, h.kp0);
-        h1[c] = a1 + ipow_rt(b1, h.kp1);
+        const Cx<T> b0 = load_cx<T>(psi + h.ob0 + c * R);
+        const Cx<T> b1 = load_cx<T>(psi + h.ob1 + c * R);
+        constexpr T sg = FWD ? T(1) : T(-1);
+        h0[c] = add_rot(a0, b0, sg * h.cp0, sg * h.sp0);
+        h1[c] = add_rot(a1, b1, sg * h.cp1, sg * h.sp1);
     }
 #pragma unroll
     for (int a = 0; a < NC; ++a) {
@@ -54,9 +94,9 @@ __device__ __forceinline__ void hop_rt(Cx<T> (&acc)[4][NC], const CS *__restrict
         }
         acc[0][a] = acc[0][a] + s0;
         acc[1][a] = acc[1][a] + s1;
-        const Cx<T> r0 = ipow_rt(s0, h.kr0), r1 = ipow_rt(s1, h.kr1);
-        acc[2][a] = acc[2][a] + (h.m0_is_2 ? r0 : r1);
-        acc[3][a] = acc[3][a] + (h.m0_is_2 ? r1 : r0);
+        constexpr T sg = FWD ? T(1) : T(-1);
+        acc[2][a] = add_rot(acc[2][a], s0, sg * h.cr0, sg * h.sr0);
+        acc[3][a] = add_rot(acc[3][a], s1, sg * h.cr1, sg * h.sr1);
     }
 }
 
@@ -75,7 +115,9 @@ __device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS
     const long site_len = 4L * NC * R;
     const long link_len = long(NC) * NC;
     const long Vh = g.Vh;
-    const HopInfo hf = hop_info(mu, true, DAG), hb = hop_info(mu, false, DAG);
+    const HopInfo hif = hop_info(mu, true, DAG);
+    const HopMul<T> hm = hop_mul<T, NC>(hif, R);  // forward; backward = negated factors
+    const bool m02 = hif.m0_is_2;  // the same for both hops of a direction
     const NbConst ncf = nb_const(g, mu, true), ncb = nb_const(g, mu, false);
     const bool lo = lane < 32, odd = (mu & 1) != 0;
 
@@ -97,8 +139,17 @@ __device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS
 #pragma unroll
                 for (int c = 0; c < NC; ++c) acc[sp][c] = {T(0), T(0)};
             if (act) {
-                hop_rt<T, CS, NC, true>(acc, in + nf * site_len + r, R, Uf, hf);
-                hop_rt<T, CS, NC, false>(acc, in + nb * site_len + r, R, Ub, hb);
+                hop_rt<T, CS, NC, true>(acc, in + nf * site_len + r, R, Uf, hm);
+                hop_rt<T, CS, NC, false>(acc, in + nb * site_len + r, R, Ub, hm);
+            }
+            // reconstructed rows -> spins 2, 3: row 0 pairs with spin 2 for z, t hops, with spin 3 for x, y
+            if (!m02) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const Cx<T> t = acc[2][c];
+                    acc[2][c] = acc[3][c];
+                    acc[3][c] = t;
+                }
             }
             // lanes < 32 keep spins {0,1}, lanes >= 32 keep spins {2,3}
             Cx<T> kept[2][NC];
