@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     }
 }
 
-template <typename S, int NC>
+template <typename S, int NC, bool WIDE>
 __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ q, const uint32_t *__restrict__ gauge,
                                                     const Cx<float> *__restrict__ m0, const int *__restrict__ walk1,
@@ -248,13 +248,23 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
         const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
         const int nb = (g & 2) ? hi : lo;
         const unsigned va = unsigned(nb) * site_bytes + rcb;
-        // field h of q at the neighbour: rows (spin, colour c'), c' < NC (padding words are zero)
-        const auto r_q = rsrc(q + size_t(h) * (fbytes / 4u), fbytes);
+        // field h of q at the neighbour: rows (spin, colour c'), c' < NC (padding words are zero).
+        // WIDE: one buffer resource spans the step's 4 fields, uniform over the wavefront, and the
+        // lane group's field is a byte offset; otherwise a resource per field, which differs between
+        // lane groups and makes the compiler loop over the distinct resources for every load.
+        __amdgpu_buffer_rsrc_t r_q;
+        unsigned vq = va;
+        if constexpr (WIDE) {
+            r_q = rsrc(q + size_t(4 * st) * (fbytes / 4u), 4u * fbytes);
+            vq = va + unsigned(g) * fbytes;
+        } else {
+            r_q = rsrc(q + size_t(h) * (fbytes / 4u), fbytes);
+        }
 #pragma unroll
         for (int sp = 0; sp < 4; ++sp)
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc)
-                qw[st][sp][cc] = cc < NC ? bload(r_q, va + unsigned(sp * NC + cc) * row_bytes, 0u) : 0u;
+                qw[st][sp][cc] = cc < NC ? bload(r_q, vq + unsigned(sp * NC + cc) * row_bytes, 0u) : 0u;
         // A word from link V[cb][c']: forward U(x)[cb][c'] (parity 1 at x), backward conj(U(nb)[c'][cb]) (parity 0)
         const unsigned gl = fwd ? (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(site)
                                 : (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(nb);
@@ -324,10 +334,11 @@ hipError_t launch_nc(void *out, const void *in, const void *gauge, void *q, cons
                        static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(gauge), frag, walk, g.Vh, R,
                        fbytes, gbytes);
     QCU_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_lgnn_gather<S, NC>), grid, block, 0, s, static_cast<uint32_t *>(out),
-                       static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(q),
-                       static_cast<const uint32_t *>(gauge), mcol + 1024, walk + size_t(g.Vh) * 9, g.Vh, R, fbytes,
-                       gbytes);
+    // four fields behind one buffer resource while 4 x fbytes stays below 2^31
+    auto gather = 4.0 * double(fbytes) < 2147483647.0 ? k_lgnn_gather<S, NC, true> : k_lgnn_gather<S, NC, false>;
+    hipLaunchKernelGGL(gather, grid, block, 0, s, static_cast<uint32_t *>(out), static_cast<const uint32_t *>(in),
+                       static_cast<const uint32_t *>(q), static_cast<const uint32_t *>(gauge), mcol + 1024,
+                       walk + size_t(g.Vh) * 9, g.Vh, R, fbytes, gbytes);
     return hipGetLastError();
 }
 
