@@ -77,6 +77,32 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, uns
     __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), 0);
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+
+// N = Nc (2..4) consecutive words in one load / store; words past N read as zero
+template <int N>
+__device__ __forceinline__ u32x4 bload_words(__amdgpu_buffer_rsrc_t r, unsigned voff) {
+    if constexpr (N == 2) {
+        const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), 0, 0);
+        return u32x4{v[0], v[1], 0u, 0u};
+    } else if constexpr (N == 3) {
+        const u32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, int(voff), 0, 0);
+        return u32x4{v[0], v[1], v[2], 0u};
+    } else {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), 0, 0);
+    }
+}
+template <int N>
+__device__ __forceinline__ void bstore_words(__amdgpu_buffer_rsrc_t r, u32x4 w, unsigned voff) {
+    if constexpr (N == 2)
+        __builtin_amdgcn_raw_buffer_store_b64(u32x2{w[0], w[1]}, r, int(voff), 0, 0);
+    else if constexpr (N == 3)
+        __builtin_amdgcn_raw_buffer_store_b96(u32x3{w[0], w[1], w[2]}, r, int(voff), 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, int(voff), 0, 0);
+}
+
 // MFMA A-operand fragments of the real 2x2 embedding of M, in lane order:
 // frag[(rb * 2 + st) * 64 + lane] = 4 words (8 values) of rows (k, s, re|im) =
 // rb*16 + lane%16 and columns (k', t, re|im) = st*32 + (lane/16)*8 + 0..7.
@@ -103,7 +129,9 @@ __global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restri
     frag[i] = a;
 }
 
-// q fields: 8 back-to-back half vectors [8][Vh][4][NC][R] of 32-bit words;
+// q fields: 8 back-to-back fields [8][Vh][spin 4][R][NC] of 32-bit words (colour
+// innermost, unlike the solver's vectors: one lane's colours are one 8/12/16-byte
+// access in both kernels, and 16 lanes cover a contiguous run);
 // `fbytes` = bytes of one field (< 2^31, checked on the host).
 template <typename S, int NC>
 __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, const uint32_t *__restrict__ in,
@@ -198,17 +226,19 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
     // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
     if (r < R) {
+        const unsigned spin_bytes = unsigned(R) * NC * 4u;
         const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
-                               unsigned(2 * (g & 1)) * NC * row_bytes + unsigned(r) * 4u;
+                               unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb) {
             const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
+            for (int i = 0; i < 2; ++i) {
+                u32x4 w = {0u, 0u, 0u, 0u};
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
-                    bstore(r_q, P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]), lbase,
-                           unsigned(i * NC + c) * row_bytes);
+                for (int c = 0; c < NC; ++c) w[c] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
+                bstore_words<NC>(r_q, w, lbase + unsigned(i) * spin_bytes);
+            }
         }
     }
 }
@@ -226,7 +256,6 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
     if (pos >= Vh) return;  // uniform
     const int r = 16 * int(blockIdx.y) + m;
     const unsigned row_bytes = unsigned(R) * 4u, site_bytes = 4u * NC * row_bytes;
-    const unsigned rcb = unsigned(min(r, R - 1)) * 4u;
     const int cb = m >> 1;                 // output colour of A row m
     const bool bre = (m & 1) == 0;
     const int cbc = cb < NC ? cb : NC - 1;  // clamped (row never stored)
@@ -239,7 +268,8 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
     f32x4 d[4];
 #pragma unroll
     for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
-    uint32_t qw[2][4][4], lw[2][4];
+    u32x4 qw[2][4];
+    uint32_t lw[2][4];
 #pragma unroll
     for (int st = 0; st < 2; ++st) {
         const int h = 4 * st + g, mu = h >> 1;
@@ -247,8 +277,8 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
         const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
         const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
         const int nb = (g & 2) ? hi : lo;
-        const unsigned va = unsigned(nb) * site_bytes + rcb;
-        // field h of q at the neighbour: rows (spin, colour c'), c' < NC (padding words are zero).
+        const unsigned va = unsigned(nb) * site_bytes + unsigned(min(r, R - 1)) * NC * 4u;
+        // field h of q at the neighbour: this lane's RHS, colours c' < NC of each spin (padding zero).
         // WIDE: one buffer resource spans the step's 4 fields, uniform over the wavefront, and the
         // lane group's field is a byte offset; otherwise a resource per field, which differs between
         // lane groups and makes the compiler loop over the distinct resources for every load.
@@ -261,10 +291,7 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
             r_q = rsrc(q + size_t(h) * (fbytes / 4u), fbytes);
         }
 #pragma unroll
-        for (int sp = 0; sp < 4; ++sp)
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc)
-                qw[st][sp][cc] = cc < NC ? bload(r_q, vq + unsigned(sp * NC + cc) * row_bytes, 0u) : 0u;
+        for (int sp = 0; sp < 4; ++sp) qw[st][sp] = bload_words<NC>(r_q, vq + unsigned(sp) * unsigned(R) * NC * 4u);
         // A word from link V[cb][c']: forward U(x)[cb][c'] (parity 1 at x), backward conj(U(nb)[c'][cb]) (parity 0)
         const unsigned gl = fwd ? (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(site)
                                 : (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(nb);
@@ -286,7 +313,7 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
         const u32x4 a = {lw[st][0], lw[st][1], lw[st][2], lw[st][3]};
 #pragma unroll
         for (int sp = 0; sp < 4; ++sp) {
-            const u32x4 b = {qw[st][sp][0], qw[st][sp][1], qw[st][sp][2], qw[st][sp][3]};
+            const u32x4 b = qw[st][sp];
             d[sp] = P::mma(a, b, d[sp]);
         }
     }
