@@ -133,6 +133,8 @@ __global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restri
 // innermost, unlike the solver's vectors: one lane's colours are one 8/12/16-byte
 // access in both kernels, and 16 lanes cover a contiguous run);
 // `fbytes` = bytes of one field (< 2^31, checked on the host).
+constexpr int kMixSites = 2;  // sites per wavefront of k_lgnn_mix
+
 template <typename S, int NC>
 __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, const uint32_t *__restrict__ in,
                                                  const uint32_t *__restrict__ gauge, const u32x4 *__restrict__ frag,
@@ -142,8 +144,8 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, g = lane >> 4;
-    const int pos = xcd_remap(blockIdx.x, gridDim.x) * 4 + wv;
-    if (pos >= Vh) return;  // uniform
+    const int pos0 = (xcd_remap(blockIdx.x, gridDim.x) * 4 + wv) * kMixSites;
+    if (pos0 >= Vh) return;  // uniform
     const int r = 16 * int(blockIdx.y) + m;
     const unsigned row_bytes = unsigned(R) * 4u, site_bytes = 4u * NC * row_bytes;
     const unsigned rcb = unsigned(min(r, R - 1)) * 4u;
@@ -157,87 +159,92 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
 #pragma unroll
         for (int st = 0; st < 2; ++st) A[rb][st] = frag[(rb * 2 + st) * 64 + lane];
 
-    const int *row = walk0 + long(pos) * 9;
-    const int site = row[0];
-    // this lane's hop in each k step: k' = 4 st + g
-    uint32_t psi[2][4][NC], lk[2][NC][NC];
+    // kMixSites consecutive sites per wavefront share the A fragments loaded above
+    for (int j = 0; j < kMixSites; ++j) {
+        const int pos = pos0 + j;
+        if (pos >= Vh) break;  // uniform
+        const int *row = walk0 + long(pos) * 9;
+        const int site = row[0];
+        // this lane's hop in each k step: k' = 4 st + g
+        uint32_t psi[2][4][NC], lk[2][NC][NC];
 #pragma unroll
-    for (int st = 0; st < 2; ++st) {
-        const int h = 4 * st + g, mu = h >> 1;
-        const bool fwd = (h & 1) == 0;
-        const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
-        const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
-        const int nb = (g & 2) ? hi : lo;
-        const unsigned va = unsigned(nb) * site_bytes + rcb;
+        for (int st = 0; st < 2; ++st) {
+            const int h = 4 * st + g, mu = h >> 1;
+            const bool fwd = (h & 1) == 0;
+            const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
+            const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
+            const int nb = (g & 2) ? hi : lo;
+            const unsigned va = unsigned(nb) * site_bytes + rcb;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int b = 0; b < NC; ++b) psi[st][t][b] = bload(r_in, va + unsigned(t * NC + b) * row_bytes, 0u);
-        // forward: U_mu(y) at parity 0; backward: U_mu(nb) at parity 1 (used as its adjoint)
-        const unsigned gl = fwd ? (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(site)
-                                : (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(nb);
+                for (int b = 0; b < NC; ++b) psi[st][t][b] = bload(r_in, va + unsigned(t * NC + b) * row_bytes, 0u);
+            // forward: U_mu(y) at parity 0; backward: U_mu(nb) at parity 1 (used as its adjoint)
+            const unsigned gl = fwd ? (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(site)
+                                    : (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(nb);
 #pragma unroll
-        for (int a = 0; a < NC; ++a)
+            for (int a = 0; a < NC; ++a)
 #pragma unroll
-            for (int b = 0; b < NC; ++b) lk[st][a][b] = bload(r_g, (gl * NC * NC + unsigned(a * NC + b)) * 4u, 0u);
-    }
-    const bool fwd0 = (g & 1) == 0;  // hop parity is the same in both k steps
-    const size_t fwords = fbytes / 4u;
-    f32x4 d[NC][4];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) d[c][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // word permutations: swap the two 16-bit halves; sign of the high / low half
-    constexpr uint32_t kSwap = 0x01000302u, kSignHi = 0x80000000u, kSignLo = 0x00008000u;
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-        // u = V psi with V[c][b] = U[c][b] forward, conj(U[b][c]) backward, as two dot2 per term:
-        //   forward:  re = (Ur, -Ui).(pr, pi), im = (Ui, Ur).(pr, pi)
-        //   backward: re = (Ur,  Ui).(pr, pi), im = (-Ui, Ur).(pr, pi)
-        uint32_t la[NC][NC], lb[NC][NC];
+                for (int b = 0; b < NC; ++b) lk[st][a][b] = bload(r_g, (gl * NC * NC + unsigned(a * NC + b)) * 4u, 0u);
+        }
+        const bool fwd0 = (g & 1) == 0;  // hop parity is the same in both k steps
+        const size_t fwords = fbytes / 4u;
+        f32x4 d[NC][4];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int b = 0; b < NC; ++b) {
-                const uint32_t uw = fwd0 ? lk[st][c][b] : lk[st][b][c];
-                const uint32_t sw = __builtin_amdgcn_perm(uw, uw, kSwap);
-                la[c][b] = fwd0 ? (uw ^ kSignHi) : uw;
-                lb[c][b] = fwd0 ? sw : (sw ^ kSignLo);
-            }
-        u32x4 bop[NC];
+            for (int rb = 0; rb < 4; ++rb) d[c][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // word permutations: swap the two 16-bit halves; sign of the high / low half
+        constexpr uint32_t kSwap = 0x01000302u, kSignHi = 0x80000000u, kSignLo = 0x00008000u;
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int st = 0; st < 2; ++st) {
+            // u = V psi with V[c][b] = U[c][b] forward, conj(U[b][c]) backward, as two dot2 per term:
+            //   forward:  re = (Ur, -Ui).(pr, pi), im = (Ui, Ur).(pr, pi)
+            //   backward: re = (Ur,  Ui).(pr, pi), im = (-Ui, Ur).(pr, pi)
+            uint32_t la[NC][NC], lb[NC][NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                float ur = 0.f, ui = 0.f;
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
                 for (int b = 0; b < NC; ++b) {
-                    ur = P::dot(la[c][b], psi[st][t][b], ur);
-                    ui = P::dot(lb[c][b], psi[st][t][b], ui);
+                    const uint32_t uw = fwd0 ? lk[st][c][b] : lk[st][b][c];
+                    const uint32_t sw = __builtin_amdgcn_perm(uw, uw, kSwap);
+                    la[c][b] = fwd0 ? (uw ^ kSignHi) : uw;
+                    lb[c][b] = fwd0 ? sw : (sw ^ kSignLo);
                 }
-                bop[c][t] = P::pack(ur, ui);
-            }
+            u32x4 bop[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+            for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) d[c][rb] = P::mma(A[rb][st], bop[c], d[c][rb]);
-    }
-    // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
-    // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
-    if (r < R) {
-        const unsigned spin_bytes = unsigned(R) * NC * 4u;
-        const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
-                               unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
+                for (int c = 0; c < NC; ++c) {
+                    float ur = 0.f, ui = 0.f;
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
+                    for (int b = 0; b < NC; ++b) {
+                        ur = P::dot(la[c][b], psi[st][t][b], ur);
+                        ui = P::dot(lb[c][b], psi[st][t][b], ui);
+                    }
+                    bop[c][t] = P::pack(ur, ui);
+                }
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                u32x4 w = {0u, 0u, 0u, 0u};
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-                for (int c = 0; c < NC; ++c) w[c] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
-                bstore_words<NC>(r_q, w, lbase + unsigned(i) * spin_bytes);
+                for (int rb = 0; rb < 4; ++rb) d[c][rb] = P::mma(A[rb][st], bop[c], d[c][rb]);
+        }
+        // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
+        // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
+        if (r < R) {
+            const unsigned spin_bytes = unsigned(R) * NC * 4u;
+            const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
+                                   unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    u32x4 w = {0u, 0u, 0u, 0u};
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) w[c] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
+                    bstore_words<NC>(r_q, w, lbase + unsigned(i) * spin_bytes);
+                }
             }
         }
     }
@@ -357,7 +364,8 @@ hipError_t launch_nc(void *out, const void *in, const void *gauge, void *q, cons
     auto *frag = reinterpret_cast<u32x4 *>(static_cast<char *>(q) + 8 * size_t(fbytes));
     hipLaunchKernelGGL((k_lgnn_frags<S>), dim3(2), dim3(256), 0, s, frag, mcol);
     QCU_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL((k_lgnn_mix<S, NC>), grid, block, 0, s, static_cast<uint32_t *>(q),
+    const dim3 grid_mix(div_up(g.Vh, 4 * kMixSites), div_up(R, 16));
+    hipLaunchKernelGGL((k_lgnn_mix<S, NC>), grid_mix, block, 0, s, static_cast<uint32_t *>(q),
                        static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(gauge), frag, walk, g.Vh, R,
                        fbytes, gbytes);
     QCU_HIP_TRY(hipGetLastError());
