@@ -141,6 +141,13 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
                                                  const int *__restrict__ walk0, int Vh, int R, unsigned fbytes,
                                                  unsigned gauge_bytes) {
     using P = W<S>;
+    // A operand: real embedding of M (8 fragments x 64 lanes x 16 bytes), staged once per
+    // workgroup in LDS and read per row block, so that only one row block's accumulators
+    // and A fragments are live at a time
+    __shared__ u32x4 sA[8 * 64];
+    sA[threadIdx.x] = frag[threadIdx.x];
+    sA[threadIdx.x + 256] = frag[threadIdx.x + 256];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int m = lane & 15, g = lane >> 4;
@@ -151,98 +158,83 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     const unsigned rcb = unsigned(min(r, R - 1)) * 4u;
     const auto r_in = rsrc(in, fbytes);
     const auto r_g = rsrc(gauge, gauge_bytes);
+    const size_t fwords = fbytes / 4u;
+    const unsigned spin_bytes = unsigned(R) * NC * 4u;
+    const bool fwd0 = (g & 1) == 0;  // hop parity is the same in both k steps
+    // word permutations: swap the two 16-bit halves; sign of the high / low half
+    constexpr uint32_t kSwap = 0x01000302u, kSignHi = 0x80000000u, kSignLo = 0x00008000u;
 
-    // A operand: real embedding of M (precomputed fragments, one 16-byte load each)
-    u32x4 A[4][2];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int st = 0; st < 2; ++st) A[rb][st] = frag[(rb * 2 + st) * 64 + lane];
-
-    // kMixSites consecutive sites per wavefront share the A fragments loaded above
     for (int j = 0; j < kMixSites; ++j) {
         const int pos = pos0 + j;
         if (pos >= Vh) break;  // uniform
         const int *row = walk0 + long(pos) * 9;
         const int site = row[0];
-        // this lane's hop in each k step: k' = 4 st + g
-        uint32_t psi[2][4][NC], lk[2][NC][NC];
+        // B operand of both k steps: this lane's hop k' = 4 st + g transported, (spin, re|im) packed
+        u32x4 bop[2][NC];
 #pragma unroll
         for (int st = 0; st < 2; ++st) {
             const int h = 4 * st + g, mu = h >> 1;
-            const bool fwd = (h & 1) == 0;
             const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
             const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
             const int nb = (g & 2) ? hi : lo;
             const unsigned va = unsigned(nb) * site_bytes + rcb;
+            uint32_t psi[4][NC], lk[NC][NC];
 #pragma unroll
             for (int t = 0; t < 4; ++t)
 #pragma unroll
-                for (int b = 0; b < NC; ++b) psi[st][t][b] = bload(r_in, va + unsigned(t * NC + b) * row_bytes, 0u);
+                for (int b = 0; b < NC; ++b) psi[t][b] = bload(r_in, va + unsigned(t * NC + b) * row_bytes, 0u);
             // forward: U_mu(y) at parity 0; backward: U_mu(nb) at parity 1 (used as its adjoint)
-            const unsigned gl = fwd ? (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(site)
-                                    : (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(nb);
+            const unsigned gl = fwd0 ? (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(site)
+                                     : (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(nb);
 #pragma unroll
             for (int a = 0; a < NC; ++a)
 #pragma unroll
-                for (int b = 0; b < NC; ++b) lk[st][a][b] = bload(r_g, (gl * NC * NC + unsigned(a * NC + b)) * 4u, 0u);
-        }
-        const bool fwd0 = (g & 1) == 0;  // hop parity is the same in both k steps
-        const size_t fwords = fbytes / 4u;
-        f32x4 d[NC][4];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) d[c][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // word permutations: swap the two 16-bit halves; sign of the high / low half
-        constexpr uint32_t kSwap = 0x01000302u, kSignHi = 0x80000000u, kSignLo = 0x00008000u;
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {
+                for (int b = 0; b < NC; ++b) lk[a][b] = bload(r_g, (gl * NC * NC + unsigned(a * NC + b)) * 4u, 0u);
             // u = V psi with V[c][b] = U[c][b] forward, conj(U[b][c]) backward, as two dot2 per term:
             //   forward:  re = (Ur, -Ui).(pr, pi), im = (Ui, Ur).(pr, pi)
             //   backward: re = (Ur,  Ui).(pr, pi), im = (-Ui, Ur).(pr, pi)
-            uint32_t la[NC][NC], lb[NC][NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
+            for (int c = 0; c < NC; ++c) {
+                uint32_t la[NC], lb[NC];
 #pragma unroll
                 for (int b = 0; b < NC; ++b) {
-                    const uint32_t uw = fwd0 ? lk[st][c][b] : lk[st][b][c];
+                    const uint32_t uw = fwd0 ? lk[c][b] : lk[b][c];
                     const uint32_t sw = __builtin_amdgcn_perm(uw, uw, kSwap);
-                    la[c][b] = fwd0 ? (uw ^ kSignHi) : uw;
-                    lb[c][b] = fwd0 ? sw : (sw ^ kSignLo);
+                    la[b] = fwd0 ? (uw ^ kSignHi) : uw;
+                    lb[b] = fwd0 ? sw : (sw ^ kSignLo);
                 }
-            u32x4 bop[NC];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
+                for (int t = 0; t < 4; ++t) {
                     float ur = 0.f, ui = 0.f;
 #pragma unroll
                     for (int b = 0; b < NC; ++b) {
-                        ur = P::dot(la[c][b], psi[st][t][b], ur);
-                        ui = P::dot(lb[c][b], psi[st][t][b], ui);
+                        ur = P::dot(la[b], psi[t][b], ur);
+                        ui = P::dot(lb[b], psi[t][b], ui);
                     }
-                    bop[c][t] = P::pack(ur, ui);
+                    bop[st][c][t] = P::pack(ur, ui);
                 }
-#pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) d[c][rb] = P::mma(A[rb][st], bop[c], d[c][rb]);
+            }
         }
         // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
         // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
-        if (r < R) {
-            const unsigned spin_bytes = unsigned(R) * NC * 4u;
-            const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
-                                   unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
+        const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
+                               unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
+        for (int rb = 0; rb < 4; ++rb) {
+            const u32x4 a0 = sA[(rb * 2) * 64 + lane], a1 = sA[(rb * 2 + 1) * 64 + lane];
+            f32x4 d[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                d[c] = P::mma(a0, bop[0][c], f32x4{0.f, 0.f, 0.f, 0.f});
+                d[c] = P::mma(a1, bop[1][c], d[c]);
+            }
+            if (r < R) {
                 const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
                     u32x4 w = {0u, 0u, 0u, 0u};
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) w[c] = P::pack(d[c][rb][2 * i], d[c][rb][2 * i + 1]);
+                    for (int c = 0; c < NC; ++c) w[c] = P::pack(d[c][2 * i], d[c][2 * i + 1]);
                     bstore_words<NC>(r_q, w, lbase + unsigned(i) * spin_bytes);
                 }
             }
