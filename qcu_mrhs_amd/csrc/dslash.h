@@ -50,6 +50,16 @@ hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStrea
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 bool dslash_mfma_supported(int Nc, int prec);
 
+// Second hop of the FP64 Schur operator fused with the reliable-update residual
+// (wave-per-site kernel, dagger 0): with v = a.xpay + a.coef D a.in on parity
+// a.parity, writes lo = scale[r] (b - v) in storage lo_prec (single, half, bf16)
+// and nrm[r] = ||b - v|| (deterministic: per-workgroup partials summed in a
+// fixed order by two small launches).  v itself is not stored.  `partials` holds at least
+// dslash_residual_partials(g, R) doubles.
+long dslash_residual_partials(const Geom &g, int R);
+hipError_t launch_dslash_residual_lo(const DslashArgs &a, const void *b, void *lo, int lo_prec, const double *scale,
+                                     double *partials, long partials_cap, double *nrm, hipStream_t stream);
+
 // One launch of the wave-per-site kernel over every group of a batch (grid.y =
 // group): out/in/xpay are batch half vectors (SegTab layout), a.R is ignored,
 // the group's sites per wavefront come from tab.spw, the traversal tile from

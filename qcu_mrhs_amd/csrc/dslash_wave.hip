@@ -102,11 +102,28 @@ __device__ __forceinline__ void hop_rt(Cx<T> (&acc)[4][NC], const CS *__restrict
 
 // Workgroup body: 4 wavefronts x spw sites, starting at traversal position
 // 4 * spw * b.
-template <typename T, typename CS, int NC, bool DAG>
+// Epilogue of the plain operator: out = coef D in (+ xpay).
+struct StoreEpi {
+    static constexpr bool kResidual = false;
+};
+// Fused residual epilogue (FP64 only): v = xpay + coef D in is not stored;
+// instead d = b - v is rounded to the inner storage as lo = scale_r d, and
+// |d|^2 is summed per RHS into the workgroup's LDS row of this wavefront
+// (slot[wave][R], one writer per entry, so the order of the sums is fixed).
+template <typename LO>
+struct ResidualEpi {
+    static constexpr bool kResidual = true;
+    const Cx<double> *b;
+    LO *lo;
+    const double *scale;
+    double *slot;
+};
+
+template <typename T, typename CS, int NC, bool DAG, typename EPI = StoreEpi>
 __device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS *__restrict__ in,
                                            const CS *__restrict__ gauge, const CS *__restrict__ xpay,
                                            T coef, const Geom &g, int R, int parity, int by, int bz,
-                                           int spw) {
+                                           int spw, const EPI &ep = EPI{}) {
     const int lane = threadIdx.x & 63;
     // wavefront-uniform: the site's coordinates and neighbours go to the scalar unit
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -169,7 +186,26 @@ __device__ __forceinline__ void wave_block(int b, CS *__restrict__ out, const CS
                 const Cx<T> recv = shfl_xor_cx(send, 16);
                 fin[c] = (odd ? kept[1][c] : kept[0][c]) + recv;
             }
-            if (act) {
+            if constexpr (EPI::kResidual) {
+                double a2 = 0.0;
+                if (act) {
+                    const long base = (site * 4L + mu) * NC * R + r;
+                    const double sc = ep.scale[r];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        Cx<double> v = load_cx<double>(xpay + base + c * R);
+                        v.re = fma(coef, fin[c].re, v.re);
+                        v.im = fma(coef, fin[c].im, v.im);
+                        const Cx<double> d = ep.b[base + c * R] - v;
+                        a2 = fma(d.re, d.re, fma(d.im, d.im, a2));
+                        store_cx(ep.lo + base + c * R, sc * d);
+                    }
+                }
+                // the 4 spins of one RHS sit in lane groups mu = 0..3
+                a2 += __shfl_xor(a2, 16);
+                a2 += __shfl_xor(a2, 32);
+                if (mu == 0 && act) ep.slot[wv * R + r] += a2;
+            } else if (act) {
                 const long base = (site * 4L + mu) * NC * R + r;
                 if (xpay) {
 #pragma unroll
@@ -198,6 +234,83 @@ __global__ __launch_bounds__(256) void k_dslash_wave(CS *__restrict__ out, const
 }
 
 // every group of a batch in one launch: blockIdx.y = group
+// Fused residual form of the second Schur hop (FP64): lo = scale_r (b - (xpay + coef D in)),
+// partials[block][r] = sum over the block's sites of |b - v|^2 for RHS r.  Dynamic LDS: 4 R doubles.
+template <int NC, typename LO>
+__global__ __launch_bounds__(256) void k_dslash_wave_residual(LO *__restrict__ lo, const Cx<double> *__restrict__ in,
+                                                             const Cx<double> *__restrict__ gauge,
+                                                             const Cx<double> *__restrict__ xpay,
+                                                             const Cx<double> *__restrict__ b,
+                                                             const double *__restrict__ scale,
+                                                             double *__restrict__ partials, double coef, Geom g,
+                                                             int R, int parity, int by, int bz, int spw) {
+    extern __shared__ double slot[];
+    for (int i = threadIdx.x; i < 4 * R; i += blockDim.x) slot[i] = 0.0;
+    __syncthreads();
+    wave_block<double, Cx<double>, NC, false, ResidualEpi<LO>>(xcd_remap(blockIdx.x, gridDim.x), nullptr, in, gauge,
+                                                              xpay, coef, g, R, parity, by, bz, spw,
+                                                              ResidualEpi<LO>{b, lo, scale, slot});
+    __syncthreads();
+    for (int i = threadIdx.x; i < R; i += blockDim.x)
+        partials[long(blockIdx.x) * R + i] = (slot[i] + slot[R + i]) + (slot[2 * R + i] + slot[3 * R + i]);
+}
+
+// out[x][r] = sum of in[i][r] over rows i in [x per, (x + 1) per) (sqrt of it if `root`); the
+// workgroup's threads are (row phase, RHS) so each row is one coalesced load; fixed summation order
+__global__ __launch_bounds__(256) void k_rows_sum(double *__restrict__ out, const double *__restrict__ in, int nrows,
+                                                 int R, int per, bool root) {
+    __shared__ double red[256];
+    const int cols = R < 256 ? R : 256, rows = 256 / cols;
+    const int c = threadIdx.x % cols, rw = threadIdx.x / cols;
+    const int r = blockIdx.y * cols + c;
+    const int i0 = blockIdx.x * per, i1 = min(nrows, i0 + per);
+    double a = 0.0;
+    if (rw < rows && r < R)
+        for (int i = i0 + rw; i < i1; i += rows) a += in[long(i) * R + r];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    if (rw == 0 && r < R) {
+        double t = 0.0;
+        for (int k = 0; k < rows; ++k) t += red[k * cols + c];
+        out[long(blockIdx.x) * R + r] = root ? sqrt(t) : t;
+    }
+}
+
+constexpr int kResidualSlices = 64;  // first-level slices of the per-workgroup partials
+
+template <int NC, typename LO>
+hipError_t launch_residual_nc(const DslashArgs &a, const DslashTune &t, const void *b, void *lo,
+                              const double *scale, double *partials, double *nrm, hipStream_t stream) {
+    const int nblk = div_up(a.g.Vh, 4 * t.spw);
+    hipLaunchKernelGGL((k_dslash_wave_residual<NC, LO>), dim3(nblk), dim3(256), size_t(4) * a.R * sizeof(double),
+                       stream, static_cast<LO *>(lo), static_cast<const Cx<double> *>(a.in),
+                       static_cast<const Cx<double> *>(a.gauge), static_cast<const Cx<double> *>(a.xpay),
+                       static_cast<const Cx<double> *>(b), scale, partials, a.coef, a.g, a.R, a.parity, t.by, t.bz,
+                       t.spw);
+    QCU_HIP_TRY(hipGetLastError());
+    // partials [nblk][R] -> [kResidualSlices][R] (stored after them) -> nrm
+    double *p2 = partials + long(nblk) * a.R;
+    const int cols = a.R < 256 ? a.R : 256;
+    hipLaunchKernelGGL(k_rows_sum, dim3(kResidualSlices, div_up(a.R, cols)), dim3(256), 0, stream, p2,
+                       static_cast<const double *>(partials), nblk, a.R, div_up(nblk, kResidualSlices), false);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_rows_sum, dim3(1, div_up(a.R, cols)), dim3(256), 0, stream, nrm,
+                       static_cast<const double *>(p2), kResidualSlices, a.R, kResidualSlices, true);
+    return hipGetLastError();
+}
+
+template <typename LO>
+hipError_t launch_residual_lo_t(const DslashArgs &a, const DslashTune &t, const void *b, void *lo,
+                                const double *scale, double *partials, double *nrm, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_residual_nc<2, LO>(a, t, b, lo, scale, partials, nrm, stream);
+        case 3: return launch_residual_nc<3, LO>(a, t, b, lo, scale, partials, nrm, stream);
+        case 4: return launch_residual_nc<4, LO>(a, t, b, lo, scale, partials, nrm, stream);
+        case 8: return launch_residual_nc<8, LO>(a, t, b, lo, scale, partials, nrm, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename T, typename CS, int NC, bool DAG>
 __global__ __launch_bounds__(256) void k_dslash_wave_groups(CS *__restrict__ out, const CS *__restrict__ in,
                                                            const CS *__restrict__ gauge,
@@ -281,6 +394,32 @@ hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStrea
         case kSingle: return dispatch_nc<float, float>(a, t, stream);
         case kHalf: return dispatch_nc<float, __half>(a, t, stream);
         case kBF16: return dispatch_nc<float, __hip_bfloat16>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+long dslash_residual_partials(const Geom &g, int R) { return (long(div_up(g.Vh, 4)) + kResidualSlices) * R; }
+
+hipError_t launch_dslash_residual_lo(const DslashArgs &a, const void *b, void *lo, int lo_prec, const double *scale,
+                                     double *partials, long partials_cap, double *nrm, hipStream_t stream) {
+    if (a.prec != kDouble || a.dagger || !a.in || !a.gauge || !a.xpay || !b || !lo || !scale || !partials || !nrm ||
+        a.R < 1 || 32L * a.R > 65536L || !dslash_supported(a.Nc, a.R))
+        return hipErrorInvalidValue;
+    if (a.g.Vh <= 0) return hipSuccess;
+    // the wave kernel's launch shape for this operator (forced when the tuned variant is another)
+    DslashTune t = dslash_resolve(a);
+    if (t.variant != kDslashWaveSite) {
+        DslashArgs w = a;
+        w.tune = DslashTune{};
+        w.tune.variant = kDslashWaveSite;
+        t = dslash_resolve(w);
+    }
+    if (t.by <= 0 || t.bz <= 0 || t.spw <= 0 || a.g.L[1] % t.by || a.g.L[2] % t.bz) return hipErrorInvalidValue;
+    if ((long(div_up(a.g.Vh, 4 * t.spw)) + kResidualSlices) * a.R > partials_cap) return hipErrorInvalidValue;
+    switch (lo_prec) {
+        case kSingle: return launch_residual_lo_t<Cx<float>>(a, t, b, lo, scale, partials, nrm, stream);
+        case kHalf: return launch_residual_lo_t<Half2Store>(a, t, b, lo, scale, partials, nrm, stream);
+        case kBF16: return launch_residual_lo_t<Bf162Store>(a, t, b, lo, scale, partials, nrm, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -80,6 +80,9 @@ private:
     hipError_t ensure(int prec);
     hipError_t dslash(void *out, const void *in, const void *gauge, int prec, int parity,
                       const void *xpay, double coef, int dagger, hipStream_t s);
+    // Reliable-update residual from tmp64_ = D_eo x_o: d = b' - (x_o - kappa^2 D_oe tmp64_)
+    // (FP64), dn_ = ||d|| per RHS, r_ = dscale_[R + r] d in inner precision `prec`.
+    hipError_t residual_lo(const void *x_o, const void *g64, double kappa, int prec, hipStream_t s);
 
     Geom g_;
     int Nc_, R_;
@@ -96,6 +99,8 @@ private:
     double *dn_ = nullptr;     // [4][R] norms
     double *dscale_ = nullptr; // [2][R] per-RHS scales (accumulation, new inner residual)
     double *h_ = nullptr;      // pinned host staging [8][R]
+    double *part_ = nullptr;   // per-workgroup partial sums of the fused residual (mixed solves)
+    long part_cap_ = 0;
     ReduceWork red_;
 };
 

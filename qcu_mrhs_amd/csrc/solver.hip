@@ -23,7 +23,8 @@
 // b' - A x replaces the iterated one.  The inner vectors are kept at unit
 // norm per RHS (scale sigma = 1/||r_true||), so FP16 never leaves its range;
 // the rescale is folded into the next p update.  Convergence is only declared
-// on the FP64 true residual.
+// on the FP64 true residual.  The second FP64 hop of A x writes no vector: its
+// epilogue forms b' - A x, its per-RHS norm and the scaled inner copy directly.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -99,8 +100,8 @@ BiCGStab::BiCGStab(const Geom &g, int Nc, int R) : g_(g), Nc_(Nc), R_(R), n_(lon
 BiCGStab::~BiCGStab() {
     for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &tmp64_, &v64_})
         dfree(*q);
-    void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_;
-    dfree(a); dfree(b); dfree(c); dfree(d); dfree(e);
+    void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_, *f = part_;
+    dfree(a); dfree(b); dfree(c); dfree(d); dfree(e); dfree(f);
     if (h_) (void)hipHostFree(h_);
     reduce_work_free(red_);
 }
@@ -137,6 +138,11 @@ hipError_t BiCGStab::ensure(int prec) {
         if (err != hipSuccess) return err;
         alloc_prec_ = prec;
     }
+    if (prec >= 0 && prec != kDouble && !part_) {
+        part_cap_ = dslash_residual_partials(g_, R_);
+        part_ = static_cast<double *>(dmalloc(sizeof(double) * size_t(part_cap_), err));
+        if (err != hipSuccess) return err;
+    }
     return hipSuccess;
 }
 
@@ -162,6 +168,25 @@ hipError_t BiCGStab::apply_schur(void *out, const void *in, void *tmp, const voi
     QCU_HIP_TRY(dslash(tmp, in, gauge, prec, 0, nullptr, 1.0, dagger, s));
     QCU_HIP_TRY(dslash(out, tmp, gauge, prec, 1, in, -kappa * kappa, dagger, s));
     return hipSuccess;
+}
+
+hipError_t BiCGStab::residual_lo(const void *x_o, const void *g64, double kappa, int prec, hipStream_t s) {
+    DslashArgs a;
+    a.out = nullptr;
+    a.in = tmp64_;
+    a.gauge = g64;
+    a.xpay = x_o;
+    a.coef = -kappa * kappa;
+    a.g = g_;
+    a.Nc = Nc_;
+    a.R = R_;
+    a.parity = 1;
+    a.dagger = 0;
+    a.prec = kDouble;
+    if (32L * R_ <= 65536L)  // 4 R doubles of LDS per workgroup
+        return launch_dslash_residual_lo(a, bp_, r_, prec, dscale_ + R_, part_, part_cap_, dn_, s);
+    QCU_HIP_TRY(dslash(v64_, tmp64_, g64, kDouble, 1, x_o, -kappa * kappa, 0, s));
+    return launch_residual_lo(dn_, r_, prec, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s);
 }
 
 hipError_t BiCGStab::true_residual(const void *b_full, const void *x_full, const void *g64,
@@ -330,8 +355,9 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                 }
                 QCU_HIP_TRY(hipGetLastError());
             }
-            QCU_HIP_TRY(apply_schur(v64_, x_o, tmp64_, gauge.g64, kDouble, kappa, 0, s));
-            QCU_HIP_TRY(launch_residual_lo(dn_, r_, P, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s));
+            // FP64 Schur operator on x_o: first hop into tmp64_, second hop fused with b' - A x_o
+            QCU_HIP_TRY(dslash(tmp64_, x_o, gauge.g64, kDouble, 0, nullptr, 1.0, 0, s));
+            QCU_HIP_TRY(residual_lo(x_o, gauge.g64, kappa, P, s));
             QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
             QCU_HIP_TRY(hipStreamSynchronize(s));
             res.reliable_updates++;
@@ -370,7 +396,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                                        hipMemcpyHostToDevice, s));
             if (redo) {
                 QCU_HIP_TRY(hipMemcpyAsync(dscale_ + R_, h_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
-                QCU_HIP_TRY(launch_residual_lo(dn_, r_, P, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s));
+                QCU_HIP_TRY(residual_lo(x_o, gauge.g64, kappa, P, s));
             }
             QCU_HIP_TRY(hipMemsetAsync(xlo_, 0, bytes_lo, s));
             QCU_HIP_TRY(launch_dotc(st_ + kTmpB * R_, r0_, r_, n_, R_, P, red_, 8, s));
