@@ -45,6 +45,16 @@ int qcuBlasAxpbypcz(void *res, const void *a, const void *x, const void *b, cons
 /* res[r] = sum_i conj(a[i*R+r]) b[i*R+r]  -> R complex FP64 (device)
  * (src/qcu_blas/qcu_complex_dotc.cu) */
 int qcuBlasDotc(void *res, const void *a, const void *b, long n, int m_rhs, int precision);
+/* qcuBlasDotc without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * block partials live in a fixed 1 MiB device scratch that the library holds
+ * on every device it runs on until the process ends; m_rhs must be <= 256
+ * (larger values are rejected with kQcuErrorInvalidValue).  For m_rhs <= 32
+ * the result is bit-identical to qcuBlasDotc's; above that it uses fewer
+ * blocks (same FP64 accumulation, another summation order).  Calls share the
+ * scratch of their device: calls that overlap in time on one GPU race. */
+int qcuBlasDotcNoAlloc(void *res, const void *a, const void *b, long n, int m_rhs, int precision);
 /* res[r] = ||a_r||_2 -> R FP64 (device)  (src/qcu_blas/qcu_complex_norm.cu) */
 int qcuBlasNorm(void *res, const void *a, long n, int m_rhs, int precision);
 

@@ -100,6 +100,7 @@ def load() -> ctypes.CDLL:
         "qcuBlasAxpby": ([vp, vp, vp, vp, vp, l, i, i], i),
         "qcuBlasAxpbypcz": ([vp, vp, vp, vp, vp, vp, vp, l, i, i], i),
         "qcuBlasDotc": ([vp, vp, vp, l, i, i], i),
+        "qcuBlasDotcNoAlloc": ([vp, vp, vp, l, i, i], i),
         "qcuBlasNorm": ([vp, vp, l, i, i], i),
         "qcuColorSpinorGather": ([vp, i, ctypes.POINTER(vp), i, l, i, i], i),
         "qcuColorSpinorScatter": ([ctypes.POINTER(vp), i, vp, i, l, i, i], i),

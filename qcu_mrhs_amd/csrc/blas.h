@@ -49,6 +49,14 @@ hipError_t launch_xsay_norm(double *nrm, void *res, const void *x, const void *y
 // inner storage `lo_prec` (FP32/FP16/BF16), in one pass; d itself is not stored
 hipError_t launch_residual_lo(double *nrm, void *lo, int lo_prec, const void *b, const void *y, const double *scale,
                               long n, int R, ReduceWork &w, int slot, hipStream_t s);
+// <a,b>_r -> res[r] like launch_dotc, with the block partials in the library's
+// fixed device scratch instead of a ReduceWork (no device allocation; calls
+// on one device share the scratch, so overlapping calls race).  R <= 256;
+// for R <= kNoAllocFullRhs the block count, and so the result, is the same
+// as launch_dotc's with a ReduceWork of 1024 blocks.
+constexpr long kNoAllocScratchBytes = 1L << 20;
+constexpr int kNoAllocFullRhs = int(kNoAllocScratchBytes / (8 * 4 * 1024));
+hipError_t launch_dotc_noalloc(C64 *res, const void *a, const void *b, long n, int R, int prec, hipStream_t s);
 // ||a||_r -> res[r] (sqrt applied when take_sqrt, else squared norm)
 hipError_t launch_norm(double *res, const void *a, long n, int R, int prec, bool take_sqrt,
                        ReduceWork &w, int slot, hipStream_t s);

@@ -11,9 +11,17 @@
 // alpha = rho / <r0, v>) on device, so no host round trip is needed.  Results
 // are bit-reproducible for a given (n, R).  Accumulation is FP64 for every
 // storage precision.
+#include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "blas.h"
+
+// Device scratch of launch_dotc_noalloc: block partials [blocks][R][4] FP64.
+// Each device holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the symbol itself is a host-side stand-in).
+static __device__ __align__(256) double scratch[qcu::kNoAllocScratchBytes / sizeof(double)];
 
 namespace qcu {
 namespace {
@@ -630,6 +638,40 @@ hipError_t launch_dotc(C64 *res, const void *a, const void *b, long n, int R, in
         DotOp<CS> op{static_cast<const CS *>(a), static_cast<const CS *>(b), res};
         return run_reduce(op, n, R, w, slot, s);
     })
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+hipError_t scratch_address(double **out) {
+    int dev = 0;
+    QCU_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        QCU_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_dotc_noalloc(C64 *res, const void *a, const void *b, long n, int R, int prec, hipStream_t s) {
+    if (R < 1 || R > kReduceThreads) return hipErrorInvalidValue;
+    double *part = nullptr;
+    QCU_HIP_TRY(scratch_address(&part));
+    // every block writes all of its R partials before the finish kernel reads
+    // them, so the scratch needs no clearing between calls
+    ReduceWork w;
+    w.partials = part;
+    w.max_blocks = std::min(1024, int(kNoAllocScratchBytes / (sizeof(double) * 4 * R)));
+    return launch_dotc(res, a, b, n, R, prec, w, 0, s);
 }
 
 hipError_t launch_norm(double *res, const void *a, long n, int R, int prec, bool take_sqrt,

@@ -489,6 +489,20 @@ int qcuBlasDotc(void *res, const void *a, const void *b, long n, int m_rhs, int 
     return hip_ok(launch_dotc(static_cast<C64 *>(res), a, b, n, m_rhs, precision, g_ops_red, 14, nullptr), "dotc")
                ? 0 : g_err;
 }
+int qcuBlasDotcNoAlloc(void *res, const void *a, const void *b, long n, int m_rhs, int precision) {
+    if (!valid_storage(precision) || !res || !a || !b || n < 0 || m_rhs < 1) {
+        set_error(kQcuErrorInvalidValue, "qcuBlasDotcNoAlloc: invalid arguments");
+        return g_err;
+    }
+    if (m_rhs > kReduceThreads) {
+        set_error(kQcuErrorInvalidValue,
+                  "qcuBlasDotcNoAlloc: m_rhs %d exceeds the maximum of %d (fixed %ld-byte device scratch)", m_rhs,
+                  kReduceThreads, kNoAllocScratchBytes);
+        return g_err;
+    }
+    return hip_ok(launch_dotc_noalloc(static_cast<C64 *>(res), a, b, n, m_rhs, precision, nullptr),
+                  "dotc (no allocation)") ? 0 : g_err;
+}
 int qcuBlasNorm(void *res, const void *a, long n, int m_rhs, int precision) {
     if (!valid_storage(precision) || m_rhs < 1 || m_rhs > 256 || !ops_red_ready()) {
         set_error(kQcuErrorInvalidValue, "qcuBlasNorm: invalid arguments");
