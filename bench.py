@@ -149,10 +149,24 @@ def solve_once(d_b, d_x, R, max_iter, tol):
     return info
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, gauge, b, plain_iters):
-    """FP64 oracle (OpenMP C) on the same system: BiCGStab with 1 and 2
-    iterations; the difference is the per-iteration cost, extrapolated to the
-    GPU plain-FP64 iteration count."""
+    """FP64 oracle (OpenMP C, x86-64-v3) on the host cores of this run:
+    (1) the headline system itself, BiCGStab stopped after 1 and 2
+    iterations, the difference being the per-iteration cost, extrapolated to
+    the GPU plain-FP64 iteration count -- this is `value`;
+    (2) the 8^3 x 8, 4-RHS Haar system solved to 1e-10 in full;
+    (3) the 16^3 x 32, 12-RHS system: one dslash timed, and BiCGStab stopped
+    after 3 iterations (capped), with its per-iteration cost."""
     from oracle import oracle
 
     L, Nc, R, kappa = w["L"], w["Nc"], w["R"], w["kappa"]
@@ -165,14 +179,42 @@ def cpu_baseline(w, gauge, b, plain_iters):
         log(f"cpu oracle bicgstab {iters} iteration(s): {t[-1]:.2f} s")
     per_iter = max(t[1] - t[0], 1e-9)
     setup = max(t[0] - per_iter, 0.0)
+    del bm
+
+    small = WORKLOADS["small"]
+    gs = fields.gauge_field(small["L"], small["Nc"], small["seeds"][0], kind=small["kind"])
+    bs = fields.to_mrhs(fields.gaussian_sources(small["L"], small["Nc"], small["R"], small["seeds"][1]), small["Nc"])
+    t0 = time.perf_counter()
+    _, ok, its, _ = oracle.bicgstab(bs, gs, small["L"], small["Nc"], small["R"], small["kappa"], 3000, 1e-10)
+    small_s = time.perf_counter() - t0
+    log(f"cpu oracle {small['name']}: {small_s:.3f} s, {its} iterations, converged={ok}")
+
+    c2 = WORKLOADS["c2"]
+    g2 = fields.gauge_field(c2["L"], c2["Nc"], c2["seeds"][0], kind=c2["kind"], eps=c2["eps"])
+    b2 = fields.to_mrhs(fields.gaussian_sources(c2["L"], c2["Nc"], c2["R"], c2["seeds"][1]), c2["Nc"])
+    half = b2.size // 2
+    t0 = time.perf_counter()
+    oracle.dslash(b2[:half], g2, c2["L"], c2["Nc"], c2["R"], 1)
+    c2_dslash_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.bicgstab(b2, g2, c2["L"], c2["Nc"], c2["R"], c2["kappa"], 3, 1e-300)
+    c2_capped_s = time.perf_counter() - t0
+    log(f"cpu oracle {c2['name']}: dslash {c2_dslash_s * 1e3:.1f} ms, 3 BiCGStab iterations {c2_capped_s:.2f} s")
+    model = cpu_model()
     return {
         "value": setup + per_iter * plain_iters,
         "unit": "s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": (f"FP64 C oracle BiCGStab on the full {w['name']} system, 1 and 2 iterations "
-                   f"({t[0]:.2f} s, {t[1]:.2f} s); time-to-solution extrapolated to the GPU plain "
-                   f"FP64 iteration count ({plain_iters})"),
+        "cpu_model": model,
+        "sample": (f"FP64 C oracle (OpenMP, {oracle.num_threads()} threads, {model}) BiCGStab on the full "
+                   f"{w['name']} system, 1 and 2 iterations ({t[0]:.2f} s, {t[1]:.2f} s); time-to-solution "
+                   f"extrapolated to the GPU plain FP64 iteration count ({plain_iters})"),
+        "small_full_solve_s": small_s,
+        "small_iterations": its,
+        "c2_dslash_s": c2_dslash_s,
+        "c2_dslash_GBps": dslash_bytes(fields.half_volume(c2["L"]), c2["Nc"], c2["R"], 16) / c2_dslash_s / 1e9,
+        "c2_three_iterations_s": c2_capped_s,
     }
 
 
@@ -245,6 +287,8 @@ def main():
                          "of one half vector (counter calibration)")
     ap.add_argument("--dslash-variant", default="auto", choices=["auto", "lane_rhs", "wave", "mfma"],
                     help="dslash kernel variant for every launch (default: automatic choice)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the dslash launch-shape autotuner (built-in default shapes)")
     ap.add_argument("--dslash-precs", default="double,single,half,bf16",
                     help="storage precisions timed by the dslash measurement")
     ap.add_argument("--groups", action="store_true",
@@ -281,6 +325,14 @@ def main():
     api.check(api.lib().qcuColorSpinorGather(d_src.ptr, 2, ptrs, 2, Vh, Nc, R))
 
     api.dslash_tune(args.dslash_variant)
+    tuned = {}
+    if args.dslash_variant == "auto" and not args.no_autotune:
+        # launch shapes of the workload chosen by the library's autotuner (the built-in default
+        # unless a candidate is faster); the solves below use them through the tuning table
+        for name, p in [(n, PREC[n]) for n in args.dslash_precs.split(",") if n]:
+            best, us, recs = api.dslash_autotune(L, Nc, R, p, repeats=5)
+            tuned[name] = {"shape": best, "us": us, "default_us": recs[0]["us"], "candidates": len(recs)}
+            log(f"autotune {name}: {best} {us:.1f} us (default {recs[0]['us']:.1f} us, {len(recs)} candidates)")
     dslash = {}
     for name, p in [(n, PREC[n]) for n in args.dslash_precs.split(",") if n]:
         ms = time_dslash(L, Nc, R, d_src, d_gauge, p)
@@ -290,10 +342,12 @@ def main():
                         "TFLOPs": dslash_flops(Vh, Nc, R) / ms / 1e9}
         log(f"dslash {name}: {ms * 1e3:.1f} us, {dslash[name]['GBps']:.0f} GB/s "
             f"({100 * dslash[name]['frac_of_8TBps']:.1f}% of 8 TB/s)")
-    # plain device copy of one FP64 half vector: the streaming rate this device reaches in
-    # practice, printed beside the 8 TB/s specification peak
+    # streaming rates of one FP64 half vector in this process, printed beside the 8 TB/s
+    # specification peak: a copy (read + write) and a per-RHS norm (read only, the
+    # library's reduction); the higher of the two is the measured streaming rate
     n = Vh * 4 * Nc * R
     d_dst = api.DeviceArray(n * 16)
+    d_nrm = api.DeviceArray(8 * R)
     api.check(api.lib().qcuCopyComplexVector(d_dst.ptr, 2, d_src.ptr, 2, n))
     api.synchronize()
     api.check(api.lib().qcuTimerStart())
@@ -302,48 +356,65 @@ def main():
     ms = api.lib().qcuTimerStopMs() / 10
     copy = {"bytes_read": n * 16, "bytes_written": n * 16, "ms": ms, "GBps": 2 * n * 16 / ms / 1e6}
     log(f"copy kernel: {ms * 1e3:.1f} us, {copy['GBps']:.0f} GB/s")
+    api.check(api.lib().qcuBlasNorm(d_nrm.ptr, d_src.ptr, Vh * 4 * Nc, R, 2))
+    api.synchronize()
+    api.check(api.lib().qcuTimerStart())
+    for _ in range(10):
+        api.check(api.lib().qcuBlasNorm(d_nrm.ptr, d_src.ptr, Vh * 4 * Nc, R, 2))
+    ms = api.lib().qcuTimerStopMs() / 10
+    copy["read_ms"] = ms
+    copy["read_GBps"] = n * 16 / ms / 1e6
+    copy["stream_GBps"] = max(copy["GBps"], copy["read_GBps"])
+    log(f"read-only stream (norm): {ms * 1e3:.1f} us, {copy['read_GBps']:.0f} GB/s")
     d_dst.free()
+    d_nrm.free()
     if args.dslash_only:
-        print(json.dumps({"workload": w["name"], "dslash": dslash, "copy": copy}))
+        print(json.dumps({"workload": w["name"], "dslash": dslash, "copy": copy, "autotune": tuned}))
         return
     d_src.free()
     del half_bytes
 
     learned_methods = [m for m in args.learned.split(",") if m]
     trained = train_network(w, args) if learned_methods else None
+    from qcu_mrhs_amd import lgnn
 
     api.initGridSize(L, Nc, R, 2, 2)
     api.loadQcuGauge(d_gauge, 2)
     api.getDslash(0, 1.0 / (2.0 * kappa) - 4.0)
 
-    results = {}
-    plain = None
-    if not args.no_plain:
-        api.set_solver_options(2, args.delta, False)
-        info = solve_once(d_b, d_x, R, args.max_iter, args.tol)
-        log(f"plain FP64: {info['wall_s']:.3f} s, {info['iterations']} iterations, "
-            f"true residual {info['true_rel_residual']:.2e}")
-        plain = info
-        results["plain_fp64"] = info
-
-    timed = {}
-    plan = [(m, False) for m in args.methods.split(",") if m] + [(m, True) for m in learned_methods]
-    if trained is not None:
-        from qcu_mrhs_amd import lgnn
-        lgnn.set_weights(trained[0], trained[1])
-    for m0, use_pc in plan:
-        m = m0 + ("+learned" if use_pc else "")
-        api.set_solver_options(PREC[m0], args.delta, use_pc)
+    def timed_runs(inner, use_pc):
+        api.set_solver_options(inner, args.delta, use_pc)
         for _ in range(args.warmup):
             solve_once(d_b, d_x, R, args.max_iter, args.tol)
         runs = [solve_once(d_b, d_x, R, args.max_iter, args.tol) for _ in range(args.steps)]
         ok = all(r["converged"] and r["true_rel_residual"] <= args.tol and not r.get("fallback", 0)
                  for r in runs)
-        mean = float(np.mean([r["wall_s"] for r in runs]))
-        timed[m] = {"mean_s": mean, "ok": ok, "runs": runs}
-        log(f"mixed {m}: {mean:.3f} s/solve over {args.steps}, {runs[-1]['iterations']} iterations, "
+        return {"mean_s": float(np.mean([r["wall_s"] for r in runs])), "ok": ok, "runs": runs}
+
+    plain = None
+    if not args.no_plain:
+        # same warm-up and number of timed solves as every other method
+        plain = timed_runs(2, False)
+        last = plain["runs"][-1]
+        log(f"plain FP64: {plain['mean_s']:.3f} s/solve over {args.steps}, {last['iterations']} iterations, "
+            f"true residual {max(r['true_rel_residual'] for r in plain['runs']):.2e}")
+
+    timed = {}
+    plan = [(m, None) for m in args.methods.split(",") if m]
+    if trained is not None:
+        # the untrained start (first Neumann term) beside the trained network
+        plan += [(m, "neumann") for m in learned_methods] + [(m, "learned") for m in learned_methods]
+    for m0, pc in plan:
+        m = m0 + (f"+{pc}" if pc else "")
+        if pc == "neumann":
+            lgnn.set_weights(*lgnn.neumann_weights(kappa, channels=8))
+        elif pc == "learned":
+            lgnn.set_weights(trained[0], trained[1])
+        timed[m] = timed_runs(PREC[m0], pc is not None)
+        runs = timed[m]["runs"]
+        log(f"mixed {m}: {timed[m]['mean_s']:.3f} s/solve over {args.steps}, {runs[-1]['iterations']} iterations, "
             f"{runs[-1]['reliable_updates']} reliable updates, true residual "
-            f"{max(r['true_rel_residual'] for r in runs):.2e}, converged={ok}")
+            f"{max(r['true_rel_residual'] for r in runs):.2e}, converged={timed[m]['ok']}")
     api.finalizeQcu()
 
     good = {m: v for m, v in timed.items() if v["ok"]}
@@ -352,9 +423,11 @@ def main():
         value = good[head]["mean_s"]
         head_prec = PREC[head.split("+")[0]]
         method = (f"reliable-update mixed precision, {head.split('+')[0]} inner dslash/vectors, FP64 "
-                  f"reductions/updates" + (", learned right preconditioner" if "+learned" in head else ""))
+                  f"reductions/updates" + (", learned right preconditioner" if "+learned" in head else "")
+                  + (", untrained network (first Neumann term) as right preconditioner"
+                     if "+neumann" in head else ""))
     else:
-        head, value, head_prec = "plain_fp64", plain["wall_s"] if plain else None, 2
+        head, value, head_prec = "plain_fp64", plain["mean_s"] if plain else None, 2
         method = "plain FP64 BiCGStab"
 
     dom = {2: "double", 1: "single", 0: "half", 4: "bf16"}[head_prec]
@@ -368,11 +441,12 @@ def main():
             "kernel": f"{kname} one-parity dslash ({dom} storage)" if args.dslash_variant == "auto"
                       else f"{args.dslash_variant} dslash ({dom} storage)",
             "algorithmic_bytes": dslash[dom]["bytes"], "avg_ms": dslash[dom]["ms"],
-            "copy_GBps": copy["GBps"], "frac_of_copy": dslash[dom]["GBps"] / copy["GBps"]}
+            "copy_GBps": copy["GBps"], "stream_GBps": copy["stream_GBps"],
+            "frac_of_stream": dslash[dom]["GBps"] / copy["stream_GBps"]}
 
     cpu = None
     if not args.no_cpu and plain is not None:
-        cpu = cpu_baseline(w, gauge, b, plain["iterations"])
+        cpu = cpu_baseline(w, gauge, b, plain["runs"][-1]["iterations"])
 
     out = {
         "metric": "time-to-solution @1e-10 residual, mixed+learned vs plain FP64; dslash GB/s",
@@ -392,14 +466,17 @@ def main():
                    "tol": args.tol, "method": method, "headline": head},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "plain_fp64_s": plain["wall_s"] if plain else None,
-        "plain_fp64_iterations": plain["iterations"] if plain else None,
-        "speedup_vs_plain_fp64": (plain["wall_s"] / value) if (plain and value) else None,
+        "plain_fp64_s": plain["mean_s"] if plain else None,
+        "plain_fp64_iterations": plain["runs"][-1]["iterations"] if plain else None,
+        "speedup_vs_plain_fp64": (plain["mean_s"] / value) if (plain and value) else None,
+        "speedup_vs_plain_fp64_per_method": ({m: plain["mean_s"] / v["mean_s"] for m, v in timed.items()}
+                                             if plain else None),
         "methods": {m: {"mean_s": v["mean_s"], "ok": v["ok"], "iterations": v["runs"][-1]["iterations"],
                         "reliable_updates": v["runs"][-1]["reliable_updates"],
                         "true_rel_residual": max(r["true_rel_residual"] for r in v["runs"])}
                     for m, v in timed.items()},
         "dslash": dslash,
+        "autotune": tuned,
         "learned": ({"train_configs": args.train_configs, "train_lattice": [16, 16, 16, 32],
                      "epochs": args.train_epochs, "loss": trained[2], "train_s": trained[3]}
                     if trained is not None else None),

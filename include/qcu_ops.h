@@ -77,9 +77,10 @@ int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_
  * sites per wavefront (0 = choose).  No reference counterpart. */
 int qcuDslashTune(int variant, int by, int bz, int spw);
 
-/* Autotune the dslash for (lattice, n_color, m_rhs, precision): times every
- * kernel variant over sites-per-wavefront and traversal-tile candidates and
- * records the fastest for later automatic launches.  best[4] receives
+/* Autotune the dslash for (lattice, n_color, m_rhs, precision): times the
+ * built-in default shape and every kernel variant over sites-per-wavefront
+ * and traversal-tile candidates, and records the fastest for later automatic
+ * launches (the default stays unless a candidate is more than 2 % faster).  best[4] receives
  * {variant, by, bz, spw}, *best_us its kernel time; records (may be NULL,
  * capacity max_records, 5 doubles each: variant, by, bz, spw, us) and
  * *n_records list every candidate.  No reference counterpart. */
@@ -88,6 +89,13 @@ int qcuDslashAutotune(const int lattice[4], int n_color, int m_rhs, int precisio
 /* Enter a launch shape into the tuning table (e.g. from a saved tuning cache). */
 int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
                       int spw);
+/* Write the tuning table to a text file (one entry per line: Lx Ly Lz Lt Nc
+ * R precision variant by bz spw) / add the entries of such a file to the
+ * table (*n_loaded receives their number).  No reference counterpart; the
+ * reference's launch shapes are compile-time constants
+ * (include/qcu_wmma_constant.h:7-22). */
+int qcuDslashTuneSave(const char *path);
+int qcuDslashTuneLoad(const char *path, int *n_loaded);
 
 /* Device timing of the stateless operations: HIP events recorded on the
  * stream those operations run on.  qcuTimerStopMs waits for the stop event

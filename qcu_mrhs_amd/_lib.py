@@ -109,6 +109,8 @@ def load() -> ctypes.CDLL:
         "qcuDslashAutotune": ([ip4, i, i, i, i, ip4, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_double), i, ip4], i),
         "qcuDslashSetTuned": ([ip4, i, i, i, i, i, i, i], i),
+        "qcuDslashTuneSave": ([ctypes.c_char_p], i),
+        "qcuDslashTuneLoad": ([ctypes.c_char_p, ip4], i),
         "qcuInvertGroups": ([i, ctypes.POINTER(QcuRhsGroup), i, d, ctypes.POINTER(QcuGroupInfo)], i),
         "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),

@@ -260,6 +260,36 @@ def dslash_tune(variant="auto", by=0, bz=0, spw=0):
     check(lib().qcuDslashTune(v, int(by), int(bz), int(spw)))
 
 
+def dslash_autotune(lattice, n_color, m_rhs, precision, repeats=10):
+    """Autotune the dslash launch shape for (lattice, Nc, R, precision); the
+    chosen shape enters the tuning table.  Returns (best, best_us, records)
+    with best = {variant, by, bz, spw} and records = every timed candidate
+    (the first is the built-in default)."""
+    L4 = (ctypes.c_int * 4)(*lattice)
+    best = (ctypes.c_int * 4)()
+    us = ctypes.c_double(0.0)
+    cap = 512
+    rec = (ctypes.c_double * (5 * cap))()
+    n = ctypes.c_int(0)
+    check(lib().qcuDslashAutotune(L4, int(n_color), int(m_rhs), int(precision), int(repeats), best,
+                                  ctypes.byref(us), rec, cap, ctypes.byref(n)))
+    names = ("variant", "by", "bz", "spw")
+    records = [dict(zip(names + ("us",), (int(rec[5 * k]), int(rec[5 * k + 1]), int(rec[5 * k + 2]),
+                                           int(rec[5 * k + 3]), float(rec[5 * k + 4]))))
+               for k in range(min(n.value, cap))]
+    return dict(zip(names, list(best))), us.value, records
+
+
+def dslash_tune_save(path):
+    check(lib().qcuDslashTuneSave(str(path).encode()))
+
+
+def dslash_tune_load(path):
+    n = ctypes.c_int(0)
+    check(lib().qcuDslashTuneLoad(str(path).encode(), ctypes.byref(n)))
+    return n.value
+
+
 def dslash_mrhs(out, inp, gauge, lattice, n_color, m_rhs, parity, dagger=0, precision=2,
                 xpay=None, coef=1.0):
     L4 = (ctypes.c_int * 4)(*lattice)

@@ -1,6 +1,10 @@
 // dslash.h -- internal launch interface of the Wilson hopping kernels.
 #pragma once
 
+#include <array>
+#include <utility>
+#include <vector>
+
 #include "qcu_common.h"
 
 namespace qcu {
@@ -40,6 +44,12 @@ hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream);
 
 // resolved launch shape (after defaults / tuning table)
 DslashTune dslash_resolve(const DslashArgs &a);
+// `t` with its unset fields (variant auto, tile or sites per wavefront <= 0)
+// replaced by the built-in defaults for a's lattice, Nc, R and precision
+DslashTune dslash_fill_defaults(const DslashArgs &a, DslashTune t);
+// tuning-table lookup and listing
+bool dslash_get_tuned(const int L[4], int Nc, int R, int prec, DslashTune &t);
+std::vector<std::pair<std::array<int, 7>, DslashTune>> dslash_tuned_entries();
 // per-(lattice, Nc, R, prec) tuning table consulted when tune.variant == auto
 void dslash_set_tuned(const int L[4], int Nc, int R, int prec, const DslashTune &t);
 // process-wide override (variant != auto forces that shape for every launch)

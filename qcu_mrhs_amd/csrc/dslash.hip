@@ -13,6 +13,7 @@
 // those lanes.  The wave-per-site kernels live in dslash_wave.hip (VALU) and
 // dslash_mfma.hip (matrix cores).  Every variant has an optional epilogue
 // out = x + coef * D psi (Schur operator, full-lattice M, source preparation).
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -204,6 +205,30 @@ DslashTune dslash_resolve(const DslashArgs &a) {
             if (it != g_tuned.end()) t = it->second;
         }
     }
+    return dslash_fill_defaults(a, t);
+}
+
+bool dslash_get_tuned(const int L[4], int Nc, int R, int prec, DslashTune &t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(L[0], L[1], L[2], L[3], Nc, R, prec));
+    if (it == g_tuned.end()) return false;
+    t = it->second;
+    return true;
+}
+
+std::vector<std::pair<std::array<int, 7>, DslashTune>> dslash_tuned_entries() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::vector<std::pair<std::array<int, 7>, DslashTune>> v;
+    for (const auto &e : g_tuned) {
+        const auto &k = e.first;
+        v.push_back({{std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), std::get<4>(k),
+                      std::get<5>(k), std::get<6>(k)},
+                     e.second});
+    }
+    return v;
+}
+
+DslashTune dslash_fill_defaults(const DslashArgs &a, DslashTune t) {
     if (t.variant == kDslashAuto) t.variant = dslash_mfma_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
     if (t.variant == kDslashMfma && !dslash_mfma_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
     if (t.variant != kDslashLaneRhs) {

@@ -583,6 +583,34 @@ int qcuDslashAutotune(const int lattice[4], int n_color, int m_rhs, int precisio
     return 0;
 }
 
+int qcuDslashTuneSave(const char *path) {
+    if (!path) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashTuneSave: null path");
+        return g_err;
+    }
+    if (dslash_tune_save(path) != hipSuccess) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashTuneSave: cannot write %s", path);
+        return g_err;
+    }
+    return 0;
+}
+
+int qcuDslashTuneLoad(const char *path, int *n_loaded) {
+    int n = 0;
+    if (!path) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashTuneLoad: null path");
+        return g_err;
+    }
+    const hipError_t e = dslash_tune_load(path, n);
+    if (n_loaded) *n_loaded = n;
+    if (e != hipSuccess) {
+        set_error(kQcuErrorInvalidValue, "qcuDslashTuneLoad: cannot read %s or malformed entry after %d entries", path,
+                  n);
+        return g_err;
+    }
+    return 0;
+}
+
 int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
                       int spw) {
     if (!lattice || variant < kDslashLaneRhs || variant > kDslashMfma || by < 0 || bz < 0 || spw < 0) {

@@ -16,5 +16,9 @@ struct TuneRecord {
 // tuning table.  `log` (optional) receives every candidate's time.
 hipError_t dslash_autotune(const int L[4], int Nc, int R, int prec, int reps, DslashTune &best,
                            double &best_us, std::vector<TuneRecord> *log);
+// Tuning table <-> text file, one entry per line:
+// Lx Ly Lz Lt Nc R precision variant by bz spw
+hipError_t dslash_tune_save(const char *path);
+hipError_t dslash_tune_load(const char *path, int &n_loaded);
 
 }  // namespace qcu

@@ -1,13 +1,19 @@
 // tune.hip -- dslash launch-shape autotuner.
 //
-// For one (lattice, Nc, R, storage precision) it times every applicable
-// kernel variant over a grid of sites-per-wavefront values and (y, z)
-// traversal tiles on scratch buffers filled with bounded values, using HIP
-// events around repeated launches, and records the fastest shape in the
-// dslash tuning table (consulted whenever a launch asks for the automatic
-// shape).  Candidates are limited to shapes every kernel accepts, so the
-// tuner never launches an invalid configuration.
+// For one (lattice, Nc, R, storage precision) it times the built-in default
+// shape and every applicable kernel variant over sites-per-wavefront values
+// and (y, z) traversal tiles on scratch buffers filled with bounded values,
+// using HIP events around repeated launches, and records the fastest shape in
+// the dslash tuning table (consulted whenever a launch asks for the automatic
+// shape); the default is kept unless a candidate beats it by more than 2 %,
+// in the sweep and again when the two are re-timed alternately.
+// Candidates are limited to shapes every kernel accepts, so the tuner never
+// launches an invalid configuration.  The right-hand sides are processed in
+// chunks of 16 by every variant (16 lanes x 4 directions of the VALU
+// wavefront, the 16 columns of the MFMA tile), so the RHS chunk is not a
+// tuning dimension.  The table can be saved to and loaded from a text file.
 #include <algorithm>
+#include <cstdio>
 #include <vector>
 
 #include "dslash.h"
@@ -67,57 +73,92 @@ hipError_t dslash_autotune(const int L[4], int Nc, int R, int prec, int reps, Ds
         hipLaunchKernelGGL(k_fill_words, dim3(1024), dim3(256), 0, nullptr, static_cast<uint32_t *>(gg), gauge / 4, 29u, prec);
         err = hipDeviceSynchronize();
     }
-    // candidate shapes
-    std::vector<DslashTune> cands;
-    std::vector<int> variants = {kDslashLaneRhs, kDslashWaveSite};
-    if (dslash_mfma_supported(Nc, prec)) variants.push_back(kDslashMfma);
+    const DslashArgs base{out, in, gg, nullptr, 1.0, g, Nc, R, 1, 0, prec};
+    // The built-in default shape is the baseline and the first candidate; another shape
+    // replaces it only when it is faster by more than the timing noise (2 %).
+    const DslashTune def = dslash_fill_defaults(base, DslashTune{});
+    std::vector<DslashTune> cands{def};
+    auto add = [&](int v, int by, int bz, int spw) {
+        DslashTune t;
+        t.variant = v;
+        t.by = by;
+        t.bz = bz;
+        t.spw = spw;
+        t = dslash_fill_defaults(base, t);
+        for (const auto &c : cands)
+            if (c.variant == t.variant && c.by == t.by && c.bz == t.bz && c.spw == t.spw) return;
+        cands.push_back(t);
+    };
     std::vector<int> bys, bzs;
     divisors_upto(L[1], 12, bys);
     divisors_upto(L[2], 12, bzs);
+    const int area0 = std::max(1, def.by * def.bz);
+    std::vector<int> variants = {kDslashWaveSite};
+    if (dslash_mfma_supported(Nc, prec)) variants.push_back(kDslashMfma);
     for (int v : variants) {
-        if (v == kDslashLaneRhs) {
-            DslashTune t;
-            t.variant = v;
-            cands.push_back(t);
-            continue;
-        }
-        for (int spw : {2, 4, 8, 16}) {
+        const std::vector<int> spws = v == kDslashMfma ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2, 4, 8};
+        for (int spw : spws) {
+            // traversal tiles: square-ish (y, z) tiles from a quarter to four times the default
+            // tile area, and the untiled order
             for (int by : bys)
                 for (int bz : bzs) {
-                    // keep the search small: square-ish tiles plus the plain order
-                    if (!(by == bz || (by == L[1] && bz == L[2]) || by * 2 == bz || bz * 2 == by)) continue;
-                    DslashTune t;
-                    t.variant = v;
-                    t.by = by;
-                    t.bz = bz;
-                    t.spw = spw;
-                    cands.push_back(t);
+                    const int area = by * bz;
+                    if (area * 4 < area0 || area > area0 * 4 || by > 2 * bz || bz > 2 * by) continue;
+                    add(v, by, bz, spw);
                 }
-            DslashTune t;
-            t.variant = v;
-            t.by = L[1];
-            t.bz = L[2];
-            t.spw = spw;
-            cands.push_back(t);
+            add(v, L[1], L[2], spw);
         }
     }
+    add(kDslashLaneRhs, 0, 0, 0);
+    auto time_shape = [&](const DslashTune &t, double &us) -> hipError_t {
+        DslashArgs a = base;
+        a.tune = t;
+        QCU_HIP_TRY(launch_dslash(a, nullptr));  // warm-up (also builds the index tables)
+        us = 1e30;
+        for (int trial = 0; trial < 2; ++trial) {  // best of two trials of `reps` launches
+            QCU_HIP_TRY(hipEventRecord(e0, nullptr));
+            for (int k = 0; k < reps; ++k) QCU_HIP_TRY(launch_dslash(a, nullptr));
+            QCU_HIP_TRY(hipEventRecord(e1, nullptr));
+            QCU_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            QCU_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            us = std::min(us, 1e3 * double(ms) / reps);
+        }
+        return hipSuccess;
+    };
+    best = def;
     best_us = 1e30;
+    double def_us = 1e30;
     for (size_t i = 0; err == hipSuccess && i < cands.size(); ++i) {
-        DslashArgs a{out, in, gg, nullptr, 1.0, g, Nc, R, 1, 0, prec};
-        a.tune = cands[i];
-        err = launch_dslash(a, nullptr);  // warm up (also builds index tables)
+        double us = 0.0;
+        err = time_shape(cands[i], us);
         if (err != hipSuccess) break;
-        err = hipEventRecord(e0, nullptr);
-        for (int k = 0; k < reps && err == hipSuccess; ++k) err = launch_dslash(a, nullptr);
-        if (err == hipSuccess) err = hipEventRecord(e1, nullptr);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-        const double us = 1e3 * double(ms) / reps;
         if (log) log->push_back(TuneRecord{cands[i], us});
-        if (err == hipSuccess && us < best_us) {
+        if (i == 0) {
+            def_us = us;
+            best_us = us;
+        } else if (us < best_us && us < 0.98 * def_us) {
             best_us = us;
             best = cands[i];
+        }
+    }
+    // confirm the winner against the default, timed alternately (the sweep's single
+    // measurements carry a few percent of run-to-run noise)
+    if (err == hipSuccess && !(best.variant == def.variant && best.by == def.by && best.bz == def.bz &&
+                               best.spw == def.spw)) {
+        double tb = 1e30, td = 1e30;
+        for (int round = 0; round < 3 && err == hipSuccess; ++round) {
+            double us = 0.0;
+            err = time_shape(best, us);
+            tb = std::min(tb, us);
+            if (err == hipSuccess) err = time_shape(def, us);
+            td = std::min(td, us);
+        }
+        if (err == hipSuccess && !(tb < 0.98 * td)) {
+            best = def;
+            best_us = td;
+        } else if (err == hipSuccess) {
+            best_us = tb;
         }
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -125,6 +166,46 @@ hipError_t dslash_autotune(const int L[4], int Nc, int R, int prec, int reps, Ds
     for (void *p : {in, out, gg})
         if (p) (void)hipFree(p);
     if (err == hipSuccess) dslash_set_tuned(L, Nc, R, prec, best);
+    return err;
+}
+
+hipError_t dslash_tune_save(const char *path) {
+    FILE *f = fopen(path, "w");
+    if (!f) return hipErrorInvalidValue;
+    fprintf(f, "# libqcu dslash tuning table: Lx Ly Lz Lt Nc R precision variant by bz spw\n");
+    for (const auto &e : dslash_tuned_entries()) {
+        const auto &k = e.first;
+        fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d\n", k[0], k[1], k[2], k[3], k[4], k[5], k[6], e.second.variant,
+                e.second.by, e.second.bz, e.second.spw);
+    }
+    return fclose(f) == 0 ? hipSuccess : hipErrorInvalidValue;
+}
+
+hipError_t dslash_tune_load(const char *path, int &n_loaded) {
+    n_loaded = 0;
+    FILE *f = fopen(path, "r");
+    if (!f) return hipErrorInvalidValue;
+    char line[256];
+    hipError_t err = hipSuccess;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int v[11];
+        if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                   &v[7], &v[8], &v[9], &v[10]) != 11 ||
+            v[7] < kDslashLaneRhs || v[7] > kDslashMfma || v[8] < 0 || v[9] < 0 || v[10] < 0 ||
+            !dslash_supported(v[4], v[5])) {
+            err = hipErrorInvalidValue;
+            break;
+        }
+        DslashTune t;
+        t.variant = v[7];
+        t.by = v[8];
+        t.bz = v[9];
+        t.spw = v[10];
+        dslash_set_tuned(v, v[4], v[5], v[6], t);
+        ++n_loaded;
+    }
+    fclose(f);
     return err;
 }
 
