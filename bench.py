@@ -289,6 +289,9 @@ def main():
                     help="dslash kernel variant for every launch (default: automatic choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the dslash launch-shape autotuner (built-in default shapes)")
+    ap.add_argument("--tune-file", default="",
+                    help="dslash tuning table: loaded instead of autotuning when the file exists, "
+                         "else written after autotuning (so a profiled run can reuse the shapes)")
     ap.add_argument("--dslash-precs", default="double,single,half,bf16",
                     help="storage precisions timed by the dslash measurement")
     ap.add_argument("--groups", action="store_true",
@@ -326,13 +329,19 @@ def main():
 
     api.dslash_tune(args.dslash_variant)
     tuned = {}
-    if args.dslash_variant == "auto" and not args.no_autotune:
+    if args.tune_file and os.path.exists(args.tune_file):
+        n = api.dslash_tune_load(args.tune_file)
+        tuned["loaded"] = {"file": args.tune_file, "entries": n}
+        log(f"dslash launch shapes: {n} entries loaded from {args.tune_file}")
+    elif args.dslash_variant == "auto" and not args.no_autotune:
         # launch shapes of the workload chosen by the library's autotuner (the built-in default
         # unless a candidate is faster); the solves below use them through the tuning table
         for name, p in [(n, PREC[n]) for n in args.dslash_precs.split(",") if n]:
             best, us, recs = api.dslash_autotune(L, Nc, R, p, repeats=5)
             tuned[name] = {"shape": best, "us": us, "default_us": recs[0]["us"], "candidates": len(recs)}
             log(f"autotune {name}: {best} {us:.1f} us (default {recs[0]['us']:.1f} us, {len(recs)} candidates)")
+        if args.tune_file:
+            api.dslash_tune_save(args.tune_file)
     dslash = {}
     for name, p in [(n, PREC[n]) for n in args.dslash_precs.split(",") if n]:
         ms = time_dslash(L, Nc, R, d_src, d_gauge, p)
