@@ -160,6 +160,17 @@ int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int n_random,
  * epoch e); loss_per_epoch[e] = mean loss per pair. */
 int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_per_epoch);
 int qcuLgnnClearData(void);
+/* Training objective of qcuLgnnTrain: power = 0 (default) is the pair loss
+ * above; power = K in 1..8 is the residual-power loss
+ *     sum_r ||(1 - A N)^K w_r||^2,  w_r = v_r / ||v_r||,
+ * on a fixed complex Gaussian vector v per stored batch (A the FP32 odd Schur
+ * operator of the batch's configuration): it weights most the modes that A N
+ * leaves far from 1, which set the iteration count of the preconditioned
+ * solve.  Back-propagated through the K factors with A^+ and N^+. */
+int qcuLgnnSetObjective(int power);
+/* One forward/backward pass of the residual-power loss on the loaded gauge
+ * field for R FP32 odd vectors v: loss and gradients as for qcuLgnnLossGrad. */
+int qcuLgnnPowerLossGrad(const void *v, int power, double *loss, double *gw1, double *gw2);
 
 /* ---- batched uneven right-hand-side groups (no reference counterpart) --
  * Solves M x = b for up to 8 groups of sources of different sizes on the
@@ -188,6 +199,11 @@ typedef struct QcuGroupInfo_s {
 } QcuGroupInfo;
 
 int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, double tol, QcuGroupInfo *info);
+/* Dslash kernel the last qcuInvertGroups batch ran, as chosen by its timing
+ * of the candidates: 2 = VALU wave per site, 3 = FP32 matrix cores
+ * (v_mfma_f32_16x16x4_f32, FP32 storage); 0 before any batch, -1 without
+ * initGridSize. */
+int qcuGroupsDslashVariant(void);
 
 /* Device-memory plumbing for callers that have no other GPU runtime. */
 void *qcuDeviceMalloc(size_t bytes);

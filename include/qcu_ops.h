@@ -74,7 +74,9 @@ int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_
  * (tuning table, else MFMA for FP16/BF16 and wave-per-site VALU otherwise),
  * 1 = lane per (site, rhs), 2 = wavefront per site (VALU), 3 = wavefront per
  * site on MFMA (FP16/BF16 only); by, bz = traversal tile in y and z and spw =
- * sites per wavefront (0 = choose).  No reference counterpart. */
+ * sites per wavefront (0 = choose; for variant 3 a negative spw = a persistent
+ * grid of -spw workgroups per CU whose wavefronts stride through their XCD's
+ * share of the sites, -64 <= spw).  No reference counterpart. */
 int qcuDslashTune(int variant, int by, int bz, int spw);
 
 /* Autotune the dslash for (lattice, n_color, m_rhs, precision): times the

@@ -127,14 +127,66 @@ class Net:
         """L = sum_r ||N psi_r - e_r||^2 / ||e_r||^2 ; gradients are
         dL/dRe W + i dL/dIm W (same shapes as W1, W2)."""
         V = int(np.prod(L))
-        phi, (full, hid, hop1, hop2) = self.forward(tr, psi_odd, L, Nc, R)
+        phi, _ = self.forward(tr, psi_odd, L, Nc, R)
         tgt = target_odd.reshape(V // 2, 4, Nc, R)
         nrm = (np.abs(tgt) ** 2).sum(axis=(0, 1, 2))          # per RHS
         diff = phi - tgt
         loss = float(((np.abs(diff) ** 2).sum(axis=(0, 1, 2)) / nrm).sum())
-        # dL/dphi^* on the odd sites (zero on even)
+        gW1, gW2 = self.grad_given_dphi(tr, psi_odd, diff / nrm, L, Nc, R)
+        return loss, gW1, gW2
+
+    def adjoint_apply(self, tr, y_odd, L, Nc, R):
+        """N^+ y on the odd sites: the adjoints of the two layers in reverse
+        order (T_h^+ and W^+), on full-lattice fields, restricted to odd."""
+        V = int(np.prod(L))
+        full = np.zeros((V, 4, Nc, R), dtype=np.complex128)
+        full[V // 2:] = y_odd.reshape(V // 2, 4, Nc, R)
+        C = self.W1.shape[0]
+        dhid = []
+        for i in range(C):
+            acc = np.zeros_like(full)
+            for h in range(NHOP):
+                acc += tr.adjoint(np.einsum("ts,vtcr->vscr", np.conj(self.W2[0, i, h]), full), h)
+            dhid.append(acc)
+        out = np.zeros_like(full)
+        for o in range(C):
+            for h in range(NHOP):
+                out += tr.adjoint(np.einsum("ts,vtcr->vscr", np.conj(self.W1[o, 0, h]), dhid[o]), h)
+        return out[V // 2:].copy()
+
+    def power_loss_and_grad(self, tr, schur, v_odd, K, L, Nc, R):
+        """Residual-power loss L = sum_r ||(1 - A N)^K w_r||^2 with
+        w_r = v_r / ||v_r|| and A = schur(x, dagger) the odd Schur operator;
+        gradients by back-propagation through the K factors:
+        g_K = w_K, dL/d(N w_j)^* = -A^+ g_{j+1}, g_j = g_{j+1} - N^+ A^+ g_{j+1}."""
+        V = int(np.prod(L))
+        shape = (V // 2, 4, Nc, R)
+        v = v_odd.reshape(shape)
+        nrm = np.sqrt((np.abs(v) ** 2).sum(axis=(0, 1, 2)))
+        w = [v / nrm]
+        for _ in range(K):
+            y, _ = self.forward(tr, w[-1], L, Nc, R)
+            w.append(w[-1] - schur(y, 0).reshape(shape))
+        loss = float((np.abs(w[K]) ** 2).sum())
+        gW1 = np.zeros_like(self.W1)
+        gW2 = np.zeros_like(self.W2)
+        g = w[K]
+        for j in range(K - 1, -1, -1):
+            u = schur(g, 1).reshape(shape)
+            a1, a2 = self.grad_given_dphi(tr, w[j], -u, L, Nc, R)
+            gW1 += a1
+            gW2 += a2
+            if j > 0:
+                g = g - self.adjoint_apply(tr, u, L, Nc, R)
+        return loss, gW1, gW2
+
+    def grad_given_dphi(self, tr, psi_odd, dphi_odd, L, Nc, R):
+        """dL/dRe W + i dL/dIm W for a loss whose derivative with respect to
+        the conjugate of the network output N psi is dphi (odd sites)."""
+        V = int(np.prod(L))
+        _, (full, hid, hop1, hop2) = self.forward(tr, psi_odd, L, Nc, R)
         dphi = np.zeros((V, 4, Nc, R), dtype=np.complex128)
-        dphi[V // 2:] = diff / nrm
+        dphi[V // 2:] = dphi_odd.reshape(V // 2, 4, Nc, R)
         C = self.W1.shape[0]
         gW2 = np.zeros_like(self.W2)
         dhid = []
@@ -149,7 +201,7 @@ class Net:
         for o in range(C):
             for h in range(NHOP):
                 gW1[o, 0, h] = 2 * np.einsum("vscr,vtcr->st", dhid[o], np.conj(hop1[0][h]))
-        return loss, gW1, gW2
+        return gW1, gW2
 
     def collapsed(self):
         """(M0, M[h][h']) of the equivalent two-hop stencil on odd sites:

@@ -402,3 +402,214 @@ EXPORT void oracle_set_num_threads(int n) {
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------ */
+/* clover term                                                         */
+/* ------------------------------------------------------------------ */
+/* The reference declares kDslashClover (include/qcu_helper_macro.h:7-11) but
+ * getDslash rejects it (src/qcu_interface.cu:97-105): there is no reference
+ * algorithm to restate, so this is the textbook Sheikholeslami-Wohlert term
+ * in the kappa normalisation of the Wilson operator M = 1 - kappa D:
+ *
+ *     M_clover = A - kappa D,
+ *     A(x) = 1 + kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+ *     sigma_{mu nu} = (i/2) [gamma_mu, gamma_nu]   (the kernels' chiral basis),
+ *     F_{mu nu}(x) = ((Q - Q^+) - tr(Q - Q^+)/Nc) / (8 i),
+ *
+ * Q_{mu nu}(x) the sum of the four plaquette leaves at x in the (mu, nu) plane,
+ * each starting at x and oriented mu -> nu.  In the chiral basis sigma is block
+ * diagonal in the spin pairs {0, 1} and {2, 3}, so A(x) is two Hermitian
+ * 2Nc x 2Nc blocks, stored as clover[parity][Vh][chirality 2][2Nc][2Nc] with
+ * row/column index (spin in the pair) * Nc + colour.                        */
+static const cplx *link_at(const cplx *gauge, const int L[4], int Nc, int mu, const int c0[4]) {
+    int c[4];
+    for (int k = 0; k < 4; ++k) c[k] = ((c0[k] % L[k]) + L[k]) % L[k];
+    const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
+    return gauge + ((long)(mu * 2 + p) * oracle_half_volume(L) + cb_index(c, L)) * (long)Nc * Nc;
+}
+
+/* out = a b, a^+ b, a b^+ (Nc x Nc) */
+static void mm(cplx *out, const cplx *a, int ta, const cplx *b, int tb, int Nc) {
+    for (int i = 0; i < Nc; ++i)
+        for (int j = 0; j < Nc; ++j) {
+            cplx s = 0;
+            for (int k = 0; k < Nc; ++k) {
+                const cplx x = ta ? conj(a[k * Nc + i]) : a[i * Nc + k];
+                const cplx y = tb ? conj(b[j * Nc + k]) : b[k * Nc + j];
+                s += x * y;
+            }
+            out[i * Nc + j] = s;
+        }
+}
+
+/* product of four links with daggers: w = l0^d0 l1^d1 l2^d2 l3^d3 */
+static void path4(cplx *w, const cplx *l[4], const int d[4], int Nc, cplx *t1, cplx *t2) {
+    mm(t1, l[0], d[0], l[1], d[1], Nc);
+    /* t2 = t1 l2^d2 */
+    for (int i = 0; i < Nc; ++i)
+        for (int j = 0; j < Nc; ++j) {
+            cplx s = 0;
+            for (int k = 0; k < Nc; ++k) s += t1[i * Nc + k] * (d[2] ? conj(l[2][j * Nc + k]) : l[2][k * Nc + j]);
+            t2[i * Nc + j] = s;
+        }
+    for (int i = 0; i < Nc; ++i)
+        for (int j = 0; j < Nc; ++j) {
+            cplx s = 0;
+            for (int k = 0; k < Nc; ++k) s += t2[i * Nc + k] * (d[3] ? conj(l[3][j * Nc + k]) : l[3][k * Nc + j]);
+            w[i * Nc + j] = s;
+        }
+}
+
+static const double _Complex GAMMA_CH[4][4][4] = {
+    {{0, 0, 0, I}, {0, 0, I, 0}, {0, -I, 0, 0}, {-I, 0, 0, 0}},
+    {{0, 0, 0, -1}, {0, 0, 1, 0}, {0, 1, 0, 0}, {-1, 0, 0, 0}},
+    {{0, 0, I, 0}, {0, 0, 0, -I}, {-I, 0, 0, 0}, {0, I, 0, 0}},
+    {{0, 0, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 0}, {0, 1, 0, 0}},
+};
+
+EXPORT void oracle_clover(cplx *clov, const cplx *gauge, const int L[4], int Nc, double csw, double kappa) {
+    const long Vh = oracle_half_volume(L);
+    const int n = 2 * Nc;
+    /* sigma_{mu nu} = (i/2)[gamma_mu, gamma_nu] */
+    cplx sig[6][4][4];
+    int pl = 0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = mu + 1; nu < 4; ++nu, ++pl)
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    cplx s = 0;
+                    for (int k = 0; k < 4; ++k)
+                        s += GAMMA_CH[mu][a][k] * GAMMA_CH[nu][k][b] - GAMMA_CH[nu][a][k] * GAMMA_CH[mu][k][b];
+                    sig[pl][a][b] = 0.5 * I * s;
+                }
+#pragma omp parallel
+    {
+        cplx *Q = malloc(sizeof(cplx) * Nc * Nc), *W = malloc(sizeof(cplx) * Nc * Nc);
+        cplx *t1 = malloc(sizeof(cplx) * Nc * Nc), *t2 = malloc(sizeof(cplx) * Nc * Nc);
+        cplx *F = malloc(sizeof(cplx) * 6 * Nc * Nc);
+#pragma omp for collapse(2) schedule(static)
+        for (int p = 0; p < 2; ++p)
+            for (long i = 0; i < Vh; ++i) {
+                int x[4];
+                cb_coords(i, p, L, x);
+                int pl2 = 0;
+                for (int mu = 0; mu < 4; ++mu)
+                    for (int nu = mu + 1; nu < 4; ++nu, ++pl2) {
+                        int xm[4], xn[4], xmn[4], xpm[4], xpn[4], xpmn[4], xnpm[4];
+                        for (int k = 0; k < 4; ++k) {
+                            xm[k] = x[k] - (k == mu);
+                            xn[k] = x[k] - (k == nu);
+                            xmn[k] = x[k] - (k == mu) - (k == nu);
+                            xpm[k] = x[k] + (k == mu);
+                            xpn[k] = x[k] + (k == nu);
+                            xpmn[k] = x[k] - (k == mu) + (k == nu);  /* x - mu + nu */
+                            xnpm[k] = x[k] + (k == mu) - (k == nu);  /* x + mu - nu */
+                        }
+                        for (int e = 0; e < Nc * Nc; ++e) Q[e] = 0;
+                        /* U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ */
+                        const cplx *l1[4] = {link_at(gauge, L, Nc, mu, x), link_at(gauge, L, Nc, nu, xpm),
+                                             link_at(gauge, L, Nc, mu, xpn), link_at(gauge, L, Nc, nu, x)};
+                        const int d1[4] = {0, 0, 1, 1};
+                        path4(W, l1, d1, Nc, t1, t2);
+                        for (int e = 0; e < Nc * Nc; ++e) Q[e] += W[e];
+                        /* U_nu(x) U_mu(x+nu-mu)^+ U_nu(x-mu)^+ U_mu(x-mu) */
+                        const cplx *l2[4] = {link_at(gauge, L, Nc, nu, x), link_at(gauge, L, Nc, mu, xpmn),
+                                             link_at(gauge, L, Nc, nu, xm), link_at(gauge, L, Nc, mu, xm)};
+                        const int d2[4] = {0, 1, 1, 0};
+                        path4(W, l2, d2, Nc, t1, t2);
+                        for (int e = 0; e < Nc * Nc; ++e) Q[e] += W[e];
+                        /* U_mu(x-mu)^+ U_nu(x-mu-nu)^+ U_mu(x-mu-nu) U_nu(x-nu) */
+                        const cplx *l3[4] = {link_at(gauge, L, Nc, mu, xm), link_at(gauge, L, Nc, nu, xmn),
+                                             link_at(gauge, L, Nc, mu, xmn), link_at(gauge, L, Nc, nu, xn)};
+                        const int d3[4] = {1, 1, 0, 0};
+                        path4(W, l3, d3, Nc, t1, t2);
+                        for (int e = 0; e < Nc * Nc; ++e) Q[e] += W[e];
+                        /* U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu) U_mu(x)^+ */
+                        const cplx *l4[4] = {link_at(gauge, L, Nc, nu, xn), link_at(gauge, L, Nc, mu, xn),
+                                             link_at(gauge, L, Nc, nu, xnpm), link_at(gauge, L, Nc, mu, x)};
+                        const int d4[4] = {1, 0, 0, 1};
+                        path4(W, l4, d4, Nc, t1, t2);
+                        for (int e = 0; e < Nc * Nc; ++e) Q[e] += W[e];
+                        /* F = ((Q - Q^+) - tr(Q - Q^+)/Nc) / (8 i) */
+                        cplx tr = 0;
+                        for (int a = 0; a < Nc; ++a) tr += Q[a * Nc + a] - conj(Q[a * Nc + a]);
+                        cplx *Fp = F + pl2 * Nc * Nc;
+                        for (int a = 0; a < Nc; ++a)
+                            for (int b = 0; b < Nc; ++b) {
+                                cplx v = Q[a * Nc + b] - conj(Q[b * Nc + a]);
+                                if (a == b) v -= tr / Nc;
+                                Fp[a * Nc + b] = v / (8.0 * I);
+                            }
+                    }
+                for (int ch = 0; ch < 2; ++ch) {
+                    cplx *blk = clov + (((long)p * Vh + i) * 2 + ch) * n * n;
+                    for (int s = 0; s < 2; ++s)
+                        for (int a = 0; a < Nc; ++a)
+                            for (int t = 0; t < 2; ++t)
+                                for (int b = 0; b < Nc; ++b) {
+                                    cplx v = (s == t && a == b) ? 1.0 : 0.0;
+                                    for (int q = 0; q < 6; ++q)
+                                        v += kappa * csw * sig[q][2 * ch + s][2 * ch + t] * F[q * Nc * Nc + a * Nc + b];
+                                    blk[(s * Nc + a) * n + t * Nc + b] = v;
+                                }
+                }
+            }
+        free(Q); free(W); free(t1); free(t2); free(F);
+    }
+}
+
+/* out = A in on one parity's multi-RHS half vector, A given as the two chirality
+ * blocks per site (clover[Vh][2][2Nc][2Nc] of that parity). */
+EXPORT void oracle_clover_apply(cplx *out, const cplx *blocks, const cplx *in, long Vh, int Nc, int R) {
+    const int n = 2 * Nc;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < Vh; ++i)
+        for (int ch = 0; ch < 2; ++ch) {
+            const cplx *blk = blocks + (i * 2 + ch) * n * n;
+            for (int row = 0; row < n; ++row)
+                for (int r = 0; r < R; ++r) {
+                    cplx s = 0;
+                    for (int col = 0; col < n; ++col) {
+                        const int t = col / Nc, b = col % Nc;
+                        s += blk[row * n + col] * in[((i * 4 + 2 * ch + t) * Nc + b) * R + r];
+                    }
+                    const int sp = row / Nc, a = row % Nc;
+                    out[((i * 4 + 2 * ch + sp) * Nc + a) * R + r] = s;
+                }
+        }
+}
+
+/* inverse of each n x n block (Gauss-Jordan with partial pivoting) */
+EXPORT void oracle_clover_invert(cplx *inv, const cplx *blocks, long nblocks, int n) {
+#pragma omp parallel
+    {
+        cplx *a = malloc(sizeof(cplx) * n * n * 2);
+#pragma omp for schedule(static)
+        for (long k = 0; k < nblocks; ++k) {
+            const cplx *m = blocks + k * n * n;
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < 2 * n; ++j) a[i * 2 * n + j] = j < n ? m[i * n + j] : (j - n == i ? 1.0 : 0.0);
+            for (int c = 0; c < n; ++c) {
+                int piv = c;
+                for (int i = c + 1; i < n; ++i)
+                    if (cabs(a[i * 2 * n + c]) > cabs(a[piv * 2 * n + c])) piv = i;
+                if (piv != c)
+                    for (int j = 0; j < 2 * n; ++j) {
+                        const cplx t = a[c * 2 * n + j];
+                        a[c * 2 * n + j] = a[piv * 2 * n + j];
+                        a[piv * 2 * n + j] = t;
+                    }
+                const cplx d = a[c * 2 * n + c];
+                for (int j = 0; j < 2 * n; ++j) a[c * 2 * n + j] /= d;
+                for (int i = 0; i < n; ++i)
+                    if (i != c) {
+                        const cplx f = a[i * 2 * n + c];
+                        for (int j = 0; j < 2 * n; ++j) a[i * 2 * n + j] -= f * a[c * 2 * n + j];
+                    }
+            }
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) inv[k * n * n + i * n + j] = a[i * 2 * n + n + j];
+        }
+        free(a);
+    }
+}

@@ -112,6 +112,7 @@ def load() -> ctypes.CDLL:
         "qcuDslashTuneSave": ([ctypes.c_char_p], i),
         "qcuDslashTuneLoad": ([ctypes.c_char_p, ip4], i),
         "qcuInvertGroups": ([i, ctypes.POINTER(QcuRhsGroup), i, d, ctypes.POINTER(QcuGroupInfo)], i),
+        "qcuGroupsDslashVariant": ([], i),
         "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnApply": ([vp, vp, i], i),
@@ -120,6 +121,9 @@ def load() -> ctypes.CDLL:
         "qcuLgnnCollect": ([i, d, i, i, ctypes.c_uint, ip4], i),
         "qcuLgnnTrain": ([i, d, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnClearData": ([], i),
+        "qcuLgnnSetObjective": ([i], i),
+        "qcuLgnnPowerLossGrad": ([vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_double)], i),
         "qcuTimerStart": ([], i),
         "qcuTimerStopMs": ([], d),
     }

@@ -59,6 +59,14 @@ void dslash_set_override(const DslashTune &t);
 hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 bool dslash_mfma_supported(int Nc, int prec);
+// FP32 storage on the FP32 matrix cores (dslash_mfma32.hip), also for batched groups
+hipError_t launch_dslash_mfma32(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
+hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream);
+bool dslash_mfma32_supported(int Nc, int prec);
+// either matrix-core kernel exists for (Nc, prec)
+inline bool dslash_matrix_supported(int Nc, int prec) {
+    return dslash_mfma_supported(Nc, prec) || dslash_mfma32_supported(Nc, prec);
+}
 
 // Second hop of the FP64 Schur operator fused with the reliable-update residual
 // (wave-per-site kernel, dagger 0): with v = a.xpay + a.coef D a.in on parity

@@ -230,9 +230,11 @@ std::vector<std::pair<std::array<int, 7>, DslashTune>> dslash_tuned_entries() {
 
 DslashTune dslash_fill_defaults(const DslashArgs &a, DslashTune t) {
     if (t.variant == kDslashAuto) t.variant = dslash_mfma_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
-    if (t.variant == kDslashMfma && !dslash_mfma_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
+    if (t.variant == kDslashMfma && !dslash_matrix_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
     if (t.variant != kDslashLaneRhs) {
-        if (t.spw <= 0) t.spw = t.variant == kDslashMfma ? 1 : 4;  // measured best at 24^3x48, R = 16
+        // spw < 0 (persistent grid, -spw workgroups per CU) exists for the FP16/BF16 MFMA kernel only
+        if (t.spw == 0 || (t.spw < 0 && !(t.variant == kDslashMfma && dslash_mfma_supported(a.Nc, a.prec))))
+            t.spw = t.variant == kDslashMfma ? 1 : 4;  // measured best at 24^3x48, R = 16
         // tile the (y, z) plane so one tile slice (the t-reuse distance) stays
         // around 256 KiB: bytes per site = 4 Nc R complex numbers (measured at
         // 24^3x48, 16 RHS: best among tiles of 0.06 - 5 MiB for FP16 and FP64)
@@ -257,7 +259,8 @@ hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
     if (a.g.Vh <= 0) return hipSuccess;
     const DslashTune t = dslash_resolve(a);
     if (t.variant == kDslashWaveSite) return launch_dslash_wave(a, t, stream);
-    if (t.variant == kDslashMfma) return launch_dslash_mfma(a, t, stream);
+    if (t.variant == kDslashMfma)
+        return a.prec == kSingle ? launch_dslash_mfma32(a, t, stream) : launch_dslash_mfma(a, t, stream);
     switch (a.prec) {
         case kDouble: return dispatch_nc<double, double>(a, stream);
         case kSingle: return dispatch_nc<float, float>(a, stream);

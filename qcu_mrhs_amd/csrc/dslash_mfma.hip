@@ -25,6 +25,7 @@
 // columns beyond Nc read clamped, valid addresses and are never stored, so
 // the kernel has no divergent branches around its loads.
 #include <cstdint>
+#include <type_traits>
 
 #include "dslash_common.h"
 #include "geom.h"
@@ -40,6 +41,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename S> struct Pk;
+template <typename S> struct PkPhase;
 template <> struct Pk<__half> {
     __device__ __forceinline__ static uint32_t add(uint32_t a, uint32_t b) {
         return __builtin_bit_cast(uint32_t, __builtin_bit_cast(f16x2, a) + __builtin_bit_cast(f16x2, b));
@@ -55,6 +57,49 @@ template <> struct Pk<__half> {
                                                       c, 0, 0, 0);
     }
 };
+// Multiplication by i^k with the parity of k known at compile time (odd: the
+// (re, im) halves swap) and its sign pattern per lane.  FP16: a packed sign
+// vector, so z + i^k w is one v_pk_fma_f16 (half swap through op_sel) and
+// i^k z one v_pk_mul_f16.
+template <> struct PkPhase<__half> {
+    f16x2 sv;
+    __device__ __forceinline__ static PkPhase make(uint32_t mask) {
+        PkPhase p;
+        // (+-1, +-1) in FP16: 0x3c00 per half, the mask's sign bits on top
+        p.sv = __builtin_bit_cast(f16x2, 0x3c003c00u | (mask & 0x80008000u));
+        return p;
+    }
+    template <bool ODD>
+    __device__ __forceinline__ uint32_t fma(uint32_t w, uint32_t z) const {  // w + i^k z
+        const f16x2 zz = __builtin_bit_cast(f16x2, z);
+        const f16x2 zr = ODD ? __builtin_shufflevector(zz, zz, 1, 0) : zz;
+        return __builtin_bit_cast(uint32_t, __builtin_bit_cast(f16x2, w) + sv * zr);
+    }
+    template <bool ODD>
+    __device__ __forceinline__ uint32_t mul(uint32_t z) const {  // i^k z
+        const f16x2 zz = __builtin_bit_cast(f16x2, z);
+        const f16x2 zr = ODD ? __builtin_shufflevector(zz, zz, 1, 0) : zz;
+        return __builtin_bit_cast(uint32_t, sv * zr);
+    }
+};
+// BF16 (no packed BF16 arithmetic): the word operation with a compile-time byte permutation
+// (absent for even k) and the per-lane sign bits
+template <> struct PkPhase<__hip_bfloat16> {
+    uint32_t mask;
+    __device__ __forceinline__ static PkPhase make(uint32_t m) {
+        PkPhase p;
+        p.mask = m;
+        return p;
+    }
+    template <bool ODD>
+    __device__ __forceinline__ uint32_t mul(uint32_t z) const {
+        if constexpr (ODD) z = __builtin_amdgcn_perm(z, z, 0x01000302u);
+        return z ^ mask;
+    }
+    template <bool ODD>
+    __device__ __forceinline__ uint32_t fma(uint32_t w, uint32_t z) const;
+};
+
 template <> struct Pk<__hip_bfloat16> {
     __device__ __forceinline__ static float re(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
     __device__ __forceinline__ static float im(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
@@ -70,6 +115,11 @@ template <> struct Pk<__hip_bfloat16> {
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
     }
 };
+
+template <bool ODD>
+__device__ __forceinline__ uint32_t PkPhase<__hip_bfloat16>::fma(uint32_t w, uint32_t z) const {
+    return Pk<__hip_bfloat16>::add(w, mul<ODD>(z));
+}
 
 constexpr int kRawBufferFlags = 0x00020000;  // gfx9 raw buffer resource, dword 3
 
@@ -95,6 +145,9 @@ __device__ __forceinline__ WordOp word_op(int k) {
     w.mask = ((odd ^ hi) << 15) | (hi << 31);
     return w;
 }
+// directions in the order x, z, y, t: those with odd phases (x, z) first
+__host__ __device__ constexpr int mu_of(int i) { return ((i & 1) << 1) | (i >> 1); }
+
 __device__ __forceinline__ uint32_t apply(const WordOp &o, uint32_t w) {
     return __builtin_amdgcn_perm(w, w, o.sel) ^ o.mask;
 }
@@ -103,7 +156,8 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, uns
     __builtin_amdgcn_raw_buffer_store_b32(int(v), r, int(voff), int(soff), 0);
 }
 
-// One wavefront = `spw` consecutive traversal positions x one 16-RHS chunk
+// One wavefront = a run of traversal positions (`spw` consecutive ones, or a
+// stride through its XCD's share in the persistent form) x one 16-RHS chunk
 // (blockIdx.y).  Per site the wavefront needs one scalar row of the walk
 // table (site + 8 neighbours), prefetched one site ahead, so the only exposed
 // memory latency per site is that of its link and spinor loads, which are
@@ -144,22 +198,27 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
     const auto r_out = rsrc(out, vec_bytes);
     const auto r_x = rsrc(xpay, xpay ? vec_bytes : 0u);
 
-    // per-lane, per-step hop constants
+    // per-lane, per-step hop constants.  The hops are taken in the direction order x, z, y, t
+    // (position hh = st * HPS + grp / GPH, direction mu_of(hh / 2)): the x and z hops have odd
+    // projection and reconstruction phases i^k, the y and t hops even ones, so every step's
+    // phases share their parity and the half swap of an odd phase is fixed at compile time.
+    using PH = PkPhase<S>;
     int mu_s[STEPS];
     bool fwd_s[STEPS];
-    WordOp p0_s[STEPS], p1_s[STEPS], r0_s[STEPS], r1_s[STEPS], bop_s[STEPS];
+    PH p0_s[STEPS], p1_s[STEPS], r0_s[STEPS], r1_s[STEPS];
+    WordOp bop_s[STEPS];
 #pragma unroll
     for (int st = 0; st < STEPS; ++st) {
-        const int h = st * HPS + grp / GPH;
-        const int mu = h >> 1;
-        const bool fwd = (h & 1) == 0;
+        const int hh = st * HPS + grp / GPH;
+        const int mu = mu_of(hh >> 1);
+        const bool fwd = (hh & 1) == 0;
         const HopInfo hi = hop_info(mu, fwd, DAG);
         mu_s[st] = mu;
         fwd_s[st] = fwd;
-        p0_s[st] = word_op(hi.kp0);
-        p1_s[st] = word_op(hi.kp1);
-        r0_s[st] = word_op(hi.kr0);
-        r1_s[st] = word_op(hi.kr1);
+        p0_s[st] = PH::make(word_op(hi.kp0).mask);
+        p1_s[st] = PH::make(word_op(hi.kp1).mask);
+        r0_s[st] = PH::make(word_op(hi.kr0).mask);
+        r1_s[st] = PH::make(word_op(hi.kr1).mask);
         // B word from the stored link word U: forward V = U[cb][c'], backward
         // V = conj(U[c'][cb]); real column (Vr, -Vi), imaginary column (Vi, Vr)
         bop_s[st].sel = bre ? kPermId : kPermRot;
@@ -167,33 +226,52 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
     }
 
     const int *wtab = walk + long(parity) * Vh * 9;
-    const int q0 = (b * 4 + wv) * spw;
-    if (q0 >= Vh) return;  // uniform
+    // Traversal positions of this wavefront: q_first, q_first + q_step, ... below q_end.
+    // spw > 0: `spw` consecutive positions.  spw < 0 (persistent grid of -spw workgroups per
+    // CU, a multiple of the 8 XCDs): the logical workgroups of one XCD (consecutive after
+    // xcd_remap) share that XCD's contiguous eighth of the traversal and stride through it
+    // together, so the sites in flight on an XCD stay one compact window while each
+    // wavefront's set-up above is paid once for many sites.
+    int q_first, q_step, q_end;
+    if (spw > 0) {
+        q_first = (b * 4 + wv) * spw;
+        q_step = 1;
+        q_end = min(q_first + spw, Vh);
+    } else {
+        const int wpx = int(gridDim.x) * 4 / kNumXcd;  // wavefronts per XCD
+        const int w = b * 4 + wv;
+        const int x = w / wpx;
+        const int chunk = (Vh + kNumXcd - 1) / kNumXcd;
+        q_first = x * chunk + (w - x * wpx);
+        q_step = wpx;
+        q_end = min((x + 1) * chunk, Vh);
+    }
+    if (q_first >= q_end) return;  // uniform
     int cur[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) cur[k] = wtab[long(q0) * 9 + k];
+    for (int k = 0; k < 9; ++k) cur[k] = wtab[long(q_first) * 9 + k];
 
-    for (int j = 0; j < spw; ++j) {
-        const int q = q0 + j;
-        if (q >= Vh) break;  // uniform
-        const bool more = (j + 1 < spw) && (q + 1 < Vh);
+    for (int q = q_first; q < q_end; q += q_step) {
+        const bool more = q + q_step < q_end;
         const int site = cur[0];
 
         unsigned voff_a[STEPS], voff_b0[STEPS], voff_b1[STEPS], gaddr[STEPS], gstep[STEPS];
 #pragma unroll
         for (int st = 0; st < STEPS; ++st) {
             // this lane's hop: two-level select of the scalar neighbour row (no branches)
+            // (walk row: cur[1 + 2 mu + (0 forward | 1 backward)])
             int nb;
             if constexpr (HPS == 4) {
-                const int lo = gb0 ? cur[2 + 4 * st] : cur[1 + 4 * st];
-                const int hi = gb0 ? cur[4 + 4 * st] : cur[3 + 4 * st];
+                // lane groups 0, 1: direction mu_of(2 st) (x | y), groups 2, 3: mu_of(2 st + 1) (z | t)
+                const int lo = gb0 ? cur[2 + 2 * st] : cur[1 + 2 * st];
+                const int hi = gb0 ? cur[6 + 2 * st] : cur[5 + 2 * st];
                 nb = gb1 ? hi : lo;
             } else {
-                nb = gb1 ? cur[2 + 2 * st] : cur[1 + 2 * st];
+                nb = gb1 ? cur[2 + 2 * mu_of(st)] : cur[1 + 2 * mu_of(st)];
             }
             const unsigned va = unsigned(nb) * site_bytes + rcb + unsigned(csub) * row_bytes;
             // spin m(0) is 3 for x, y hops and 2 for z, t hops: lower-spin rows of the projection
-            const bool m02 = (st * HPS) >= 4;
+            const bool m02 = mu_s[st] >= 2;
             voff_a[st] = va;
             voff_b0[st] = va + (m02 ? 0u : unsigned(NC) * row_bytes);
             voff_b1[st] = va + (m02 ? unsigned(NC) * row_bytes : 0u);
@@ -206,7 +284,7 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         int nxt[9];
         if (more) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) nxt[k] = wtab[long(q + 1) * 9 + k];
+            for (int k = 0; k < 9; ++k) nxt[k] = wtab[long(q + q_step) * 9 + k];
         }
         // every load of this site first: spinor rows, then links
         uint32_t w0[STEPS][4], w1[STEPS][4], wb0[STEPS][4], wb1[STEPS][4], lw[STEPS][4];
@@ -248,13 +326,24 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int st = 0; st < STEPS; ++st) {
-            const bool m02 = (st * HPS) >= 4;
+            constexpr bool kOddFirst = true;  // steps over x and z hops come first
+            const bool odd = kOddFirst && (st * HPS < 4);
+            const bool m02 = mu_s[st] >= 2;  // per lane when a step holds two directions (HPS = 4)
             u32x4 a[4], bf;
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
-                const uint32_t h0 = P::add(w0[st][cc], apply(p0_s[st], wb0[st][cc]));
-                const uint32_t h1 = P::add(w1[st][cc], apply(p1_s[st], wb1[st][cc]));
-                const uint32_t g0 = apply(r0_s[st], h0), g1 = apply(r1_s[st], h1);
+                uint32_t h0, h1, g0, g1;
+                if (odd) {
+                    h0 = p0_s[st].template fma<true>(w0[st][cc], wb0[st][cc]);
+                    h1 = p1_s[st].template fma<true>(w1[st][cc], wb1[st][cc]);
+                    g0 = r0_s[st].template mul<true>(h0);
+                    g1 = r1_s[st].template mul<true>(h1);
+                } else {
+                    h0 = p0_s[st].template fma<false>(w0[st][cc], wb0[st][cc]);
+                    h1 = p1_s[st].template fma<false>(w1[st][cc], wb1[st][cc]);
+                    g0 = r0_s[st].template mul<false>(h0);
+                    g1 = r1_s[st].template mul<false>(h1);
+                }
                 a[0][cc] = h0;
                 a[1][cc] = h1;
                 a[2][cc] = m02 ? g0 : g1;
@@ -268,24 +357,35 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         }
         __builtin_amdgcn_s_setprio(0);
         // lane (m, grp) holds rows 4*grp + e = (colour 2*grp + e/2, re|im) of RHS r
-        if (r < R) {
-            const unsigned obase = unsigned(site) * site_bytes + unsigned(r) * 4u + unsigned(2 * grp) * row_bytes;
+        const bool unit = coef == 1.f;  // the bare operator's first Schur hop: no scaling
+        // lane (m, grp) holds rows 4*grp + e = (colour 2*grp + e/2, re|im) of RHS r
+        auto store = [&](auto unit_c) {
+            constexpr bool kUnit = decltype(unit_c)::value;
+            if (r < R) {
+                const unsigned obase = unsigned(site) * site_bytes + unsigned(r) * 4u + unsigned(2 * grp) * row_bytes;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                if (2 * grp + i < NC) {
+                for (int i = 0; i < 2; ++i) {
+                    if (2 * grp + i < NC) {
 #pragma unroll
-                    for (int sp = 0; sp < 4; ++sp) {
-                        const unsigned so = unsigned(sp * NC + i) * row_bytes;
-                        float vr = coef * d[sp][2 * i], vi = coef * d[sp][2 * i + 1];
-                        if constexpr (XP) {
-                            vr += P::re(xw[i][sp]);
-                            vi += P::im(xw[i][sp]);
+                        for (int sp = 0; sp < 4; ++sp) {
+                            const unsigned so = unsigned(sp * NC + i) * row_bytes;
+                            float vr = d[sp][2 * i], vi = d[sp][2 * i + 1];
+                            if constexpr (!kUnit) {
+                                vr *= coef;
+                                vi *= coef;
+                            }
+                            if constexpr (XP) {
+                                vr += P::re(xw[i][sp]);
+                                vi += P::im(xw[i][sp]);
+                            }
+                            bstore(r_out, P::pack(vr, vi), obase, so);
                         }
-                        bstore(r_out, P::pack(vr, vi), obase, so);
                     }
                 }
             }
-        }
+        };
+        if (unit) store(std::true_type{});
+        else store(std::false_type{});
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) cur[k] = nxt[k];
@@ -297,7 +397,9 @@ template <typename S, int NC>
 hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(a.g, t.by, t.bz, tab, stream));
-    dim3 grid(div_up(a.g.Vh, 4 * t.spw), div_up(a.R, 16)), block(256);
+    // spw < 0: persistent grid of -spw workgroups per CU
+    const int nblk = t.spw > 0 ? div_up(a.g.Vh, 4 * t.spw) : -t.spw * num_cus();
+    dim3 grid(nblk, div_up(a.R, 16)), block(256);
     auto out = static_cast<uint32_t *>(a.out);
     auto in = static_cast<const uint32_t *>(a.in);
     auto gauge = static_cast<const uint32_t *>(a.gauge);
@@ -331,7 +433,9 @@ bool dslash_mfma_supported(int Nc, int prec) {
 
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     if (!dslash_mfma_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
-    if (t.by <= 0 || t.bz <= 0 || t.spw <= 0 || a.g.L[1] % t.by || a.g.L[2] % t.bz) return hipErrorInvalidValue;
+    if (t.by <= 0 || t.bz <= 0 || t.spw == 0 || t.spw < -64 || a.g.L[1] % t.by || a.g.L[2] % t.bz)
+        return hipErrorInvalidValue;
+    if (t.spw < 0 && (num_cus() * 4) % kNumXcd != 0) return hipErrorInvalidValue;
     // buffer resources address at most 2^31 - 1 bytes
     if (double(a.g.Vh) * 16.0 * a.Nc * a.R >= 2147483647.0 || double(a.g.Vh) * 32.0 * a.Nc * a.Nc >= 2147483647.0)
         return hipErrorInvalidValue;

@@ -50,6 +50,8 @@ public:
     // Allocates the batch buffers and picks each group's dslash shape (sites
     // per wavefront) by timing it on the inner precision; cached per precision.
     hipError_t autotune(const void *glo, int prec, hipStream_t s);
+    // dslash kernel the autotuner chose for the batch (2 = VALU wave-per-site, 3 = FP32 matrix cores)
+    int dslash_variant() const { return variant_; }
 
     // Solve M x = b for the batch held in b_full(); result in x_full().
     hipError_t solve(const void *g64, const void *glo, int inner_prec, double kappa, int max_iter, double tol,
@@ -68,6 +70,7 @@ private:
     long n_;
     SegTab tab_;
     int by_ = 0, bz_ = 0;
+    int variant_ = 2;  // group dslash kernel: 2 = wave-per-site VALU, 3 = FP32 matrix cores (autotuned)
     int lo_prec_ = -1;
     bool tuned_ = false;
     int tuned_prec_ = -1;

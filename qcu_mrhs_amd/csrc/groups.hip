@@ -96,6 +96,8 @@ hipError_t GroupSolver::ensure(int prec) {
 hipError_t GroupSolver::dslash(void *out, const void *in, const void *gauge, int prec, int parity,
                                const void *xpay, double coef, const SegTab &tab, hipStream_t s) {
     DslashArgs a{out, in, gauge, xpay, coef, g_, Nc_, 0, parity, 0, prec};
+    if (variant_ == kDslashMfma && dslash_mfma32_supported(Nc_, prec))
+        return launch_dslash_mfma32_groups(a, tab, by_, bz_, s);
     return launch_dslash_wave_groups(a, tab, by_, bz_, s);
 }
 
@@ -109,30 +111,69 @@ hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
     QCU_HIP_TRY(launch_scalar_fill(st_, C64{0.0, 0.0}, kNumSlots * tab_.Rtot, s));
     QCU_HIP_TRY(hipMemsetAsync(p_, 0, size_t(n_) * tab_.Rtot * 2 * prec_bytes(prec), s));
     hipError_t err = hipSuccess;
-    for (int k = 0; k < tab_.G && err == hipSuccess; ++k) {
-        // only group k active
-        for (int j = 0; j < kMaxGroups; ++j) reinterpret_cast<int *>(h_)[j] = (j == k) ? 0 : 1;
-        err = hipMemcpyAsync(done_, h_, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
-        if (err == hipSuccess) err = hipStreamSynchronize(s);
-        double best = 1e30;
-        int best_spw = tab_.spw[k];
-        for (int spw : {1, 2, 4, 8, 16}) {
+    // Kernel variants: the wave-per-site VALU kernel and, for FP32 storage, the FP32 matrix-core
+    // kernel.  Per variant every group is timed alone (the others flagged done) over sites per
+    // wavefront, then the whole batch with those shapes; the variant whose batch launch is
+    // faster serves all groups (one launch covers the batch).
+    std::vector<int> variants{kDslashWaveSite};
+    if (dslash_mfma32_supported(Nc_, prec)) variants.push_back(kDslashMfma);
+    double best_total = 1e30;
+    int best_variant = kDslashWaveSite;
+    int best_spw_of[kMaxGroups] = {};
+    for (int v : variants) {
+        variant_ = v;
+        double total = 0.0;
+        int spw_of[kMaxGroups] = {};
+        for (int k = 0; k < tab_.G && err == hipSuccess; ++k) {
+            // only group k active
+            for (int j = 0; j < kMaxGroups; ++j) reinterpret_cast<int *>(h_)[j] = (j == k) ? 0 : 1;
+            err = hipMemcpyAsync(done_, h_, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+            if (err == hipSuccess) err = hipStreamSynchronize(s);
+            double best = 1e30;
+            int best_spw = 1;
+            for (int spw : {1, 2, 4, 8, 16}) {
+                SegTab t = tab_with(done_);
+                t.spw[k] = spw;
+                if (err == hipSuccess) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);  // warm-up
+                if (err == hipSuccess) err = hipEventRecord(e0, s);
+                for (int rep = 0; rep < 5 && err == hipSuccess; ++rep)
+                    err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
+                if (err == hipSuccess) err = hipEventRecord(e1, s);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                if (err == hipSuccess && ms < best) {
+                    best = ms;
+                    best_spw = spw;
+                }
+            }
+            spw_of[k] = best_spw;
+            total += best;
+        }
+        // the batch as it runs: every group active, each with its own best shape
+        if (err == hipSuccess && variants.size() > 1) {
+            for (int j = 0; j < kMaxGroups; ++j) reinterpret_cast<int *>(h_)[j] = 0;
+            err = hipMemcpyAsync(done_, h_, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
             SegTab t = tab_with(done_);
-            t.spw[k] = spw;
-            if (err == hipSuccess) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);  // warm-up
+            for (int k = 0; k < tab_.G; ++k) t.spw[k] = spw_of[k];
+            if (err == hipSuccess) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
             if (err == hipSuccess) err = hipEventRecord(e0, s);
-            for (int rep = 0; rep < 5 && err == hipSuccess; ++rep) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
+            for (int rep = 0; rep < 5 && err == hipSuccess; ++rep)
+                err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
             if (err == hipSuccess) err = hipEventRecord(e1, s);
             if (err == hipSuccess) err = hipEventSynchronize(e1);
             float ms = 0.f;
             if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-            if (err == hipSuccess && ms < best) {
-                best = ms;
-                best_spw = spw;
-            }
+            total = ms;
         }
-        tab_.spw[k] = best_spw;
+        if (err == hipSuccess && total < best_total) {
+            best_total = total;
+            best_variant = v;
+            for (int k = 0; k < tab_.G; ++k) best_spw_of[k] = spw_of[k];
+        }
     }
+    variant_ = best_variant;
+    for (int k = 0; k < tab_.G; ++k) tab_.spw[k] = best_spw_of[k];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (err == hipSuccess) {

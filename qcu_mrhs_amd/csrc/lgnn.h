@@ -67,12 +67,29 @@ public:
     // Adam step with the accumulated gradient (divided by `batches`).
     hipError_t adam_step(double lr, int batches, hipStream_t s);
 
+    // Residual-power objective on a batch of R FP32 odd vectors v:
+    //     loss = sum_r ||(1 - A N)^K w_r||^2,  w_r = v_r / ||v_r||,
+    // A = 1 - kappa^2 D_oe D_eo the odd Schur operator of the current gauge field in
+    // FP32.  The K-th power weights the modes that the preconditioned operator A N
+    // leaves far from 1 -- those that set the Krylov iteration count -- ever more
+    // strongly as K grows.  Gradients as in loss_grad (accumulated).
+    hipError_t power_loss_grad(const void *v, int K, double kappa, double *loss_host, hipStream_t s);
+
 private:
     hipError_t ensure_fields(hipStream_t s);
     hipError_t ensure_work32(hipStream_t s);
     hipError_t apply_mfma(void *out, const void *in, int prec, hipStream_t s);
     hipError_t collapse(hipStream_t s);
-    hipError_t forward(const void *in, int prec, void *out, int out_prec, hipStream_t s);
+    // stencil: collapsed stencil to apply (default the network's); hops: where the 8 hop
+    // fields of the input go (default the work field u_)
+    hipError_t forward(const void *in, int prec, void *out, int out_prec, hipStream_t s,
+                       const Cx<float> *stencil = nullptr, void *hops = nullptr);
+    // gradient of a loss whose derivative with respect to the network output N w is dphi
+    // (FP32 odd field w; hops = the 8 hop fields forward(w) produced, default u_)
+    hipError_t accumulate_grad(const void *w, const void *dphi, hipStream_t s, const void *hops = nullptr);
+    // FP32 odd Schur operator with the FP32 gauge copy: out = A in (dagger: A^+)
+    hipError_t schur32(void *out, const void *in, double kappa, int dagger, hipStream_t s);
+    hipError_t ensure_power(int K, hipStream_t s);
 
     Geom g_;
     int Nc_, R_, C_;
@@ -87,6 +104,12 @@ private:
     int adam_t_ = 0;
     // collapsed stencil (FP32 complex): M0 [16], M [64][16] as a 32 x 32 matrix
     Cx<float> *mcol_ = nullptr;                  // [32*32 + 16]
+    Cx<float> *madj_ = nullptr;                  // the stencil of N^+ (same layout)
+    // residual-power objective work fields (FP32 odd half vectors): w_0..w_K, g, A^+ g, N^+ A^+ g,
+    // the dslash intermediate on the even sites, and per-RHS scale factors
+    std::vector<void *> pw_, phops_;
+    void *pg_ = nullptr, *pu_ = nullptr, *pn_ = nullptr, *ptmp_ = nullptr;
+    double *pscale_ = nullptr;
     // work fields (FP32): u[8] hop fields on even sites, q[8], odd scratch
     void *u_ = nullptr, *q_ = nullptr, *inf_ = nullptr, *phi_ = nullptr, *dlt_ = nullptr;
     C64 *gm_ = nullptr;                          // dL/dM* partials [nblk][32*32 + 16]

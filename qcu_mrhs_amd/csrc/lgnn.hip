@@ -329,6 +329,24 @@ __global__ void k_collapse(Cx<float> *mcol, const C64 *w1, const C64 *w2, int C)
     }
 }
 
+// Stencil of N^+ from that of N: with N = M0 + sum_{k,k'} T_k M[k][k'] T_k' and T_k^+ = T_{k^1},
+// N^+ = M0^+ + sum_{a,b} T_a (M[b^1][a^1])^+ T_b, so block (a, b) of the adjoint stencil is the
+// conjugate transpose of block (b^1, a^1).
+__global__ void k_adjoint_stencil(Cx<float> *adj, const Cx<float> *m) {
+    for (int e = threadIdx.x; e < 1040; e += blockDim.x) {
+        Cx<float> v;
+        if (e < 1024) {
+            const int i = e / 32, j = e % 32;  // row (a, s), column (b, t)
+            const int a = i >> 2, sp = i & 3, b = j >> 2, t = j & 3;
+            v = m[((b ^ 1) * 4 + t) * 32 + (a ^ 1) * 4 + sp];
+        } else {
+            const int sp = (e - 1024) >> 2, t = (e - 1024) & 3;
+            v = m[1024 + t * 4 + sp];
+        }
+        adj[e] = {v.re, -v.im};
+    }
+}
+
 __global__ void k_adam(C64 *w, C64 *m, C64 *v, const C64 *g, int n, double lr, double b1, double b2, double eps,
                        double c1, double c2, double inv_batches) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -403,8 +421,13 @@ LgnnPreconditioner::LgnnPreconditioner(const Geom &g, int Nc, int R, int channel
 
 LgnnPreconditioner::~LgnnPreconditioner() {
     for (void *p : {(void *)w1_, (void *)w2_, (void *)g1_, (void *)g2_, (void *)m1_, (void *)m2_, (void *)v1_,
-                    (void *)v2_, (void *)mcol_, u_, q_, inf_, phi_, dlt_, (void *)gm_, (void *)dnorm_,
+                    (void *)v2_, (void *)mcol_, (void *)madj_, u_, q_, inf_, phi_, dlt_, (void *)gm_, (void *)dnorm_,
+                    pg_, pu_, pn_, ptmp_, (void *)pscale_,
                     (void *)dloss_, gauge32_, q16_, glo_[0], glo_[1]})
+        if (p) (void)hipFree(p);
+    for (void *p : pw_)
+        if (p) (void)hipFree(p);
+    for (void *p : phops_)
         if (p) (void)hipFree(p);
     if (hbuf_) (void)hipHostFree(hbuf_);
     reduce_work_free(red_);
@@ -434,6 +457,7 @@ hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
     hipError_t err = hipSuccess;
     const size_t wbytes = size_t(C_) * 9 * 16 * sizeof(C64);
     mcol_ = static_cast<Cx<float> *>(dmal(sizeof(Cx<float>) * 1040, err));
+    madj_ = static_cast<Cx<float> *>(dmal(sizeof(Cx<float>) * 1040, err));
     w1_ = static_cast<C64 *>(dmal(wbytes, err));
     w2_ = static_cast<C64 *>(dmal(wbytes, err));
     g1_ = static_cast<C64 *>(dmal(wbytes, err));
@@ -484,10 +508,14 @@ hipError_t LgnnPreconditioner::get_weights(double *w1, double *w2, hipStream_t s
 
 hipError_t LgnnPreconditioner::collapse(hipStream_t s) {
     hipLaunchKernelGGL(k_collapse, dim3(1), dim3(256), 0, s, mcol_, w1_, w2_, C_);
+    QCU_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_adjoint_stencil, dim3(1), dim3(256), 0, s, madj_, mcol_);
     return hipGetLastError();
 }
 
-hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int out_prec, hipStream_t s) {
+hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int out_prec, hipStream_t s,
+                                       const Cx<float> *stencil, void *hops) {
+    const Cx<float> *mc = stencil ? stencil : mcol_;
     QCU_HIP_TRY(ensure_work32(s));
     if (!in || !out || !u_ || !q_) return hipErrorInvalidValue;
     GeomTables tab;
@@ -495,7 +523,7 @@ hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int 
     const int spb = R_ >= 256 ? 1 : 256 / R_;
     const dim3 block(R_, spb), grid(div_up(g_.Vh, spb));
     const long fsize = n_ * R_;
-    C32 *u = static_cast<C32 *>(u_), *q = static_cast<C32 *>(q_);
+    C32 *u = static_cast<C32 *>(hops ? hops : u_), *q = static_cast<C32 *>(q_);
     const C32 *g32 = static_cast<const C32 *>(gauge32_);
     hipError_t e = dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
         constexpr int NC = decltype(ncv)::value;
@@ -510,10 +538,10 @@ hipError_t LgnnPreconditioner::forward(const void *in, int prec, void *out, int 
     });
     if (e != hipSuccess) return e;
     const long ncol = long(g_.Vh) * Nc_ * R_;
-    hipLaunchKernelGGL(k_mix, dim3(int(std::min<long>((ncol + 255) / 256, 8192))), dim3(256), 0, s, q, u, mcol_, ncol,
+    hipLaunchKernelGGL(k_mix, dim3(int(std::min<long>((ncol + 255) / 256, 8192))), dim3(256), 0, s, q, u, mc, ncol,
                        Nc_, R_, fsize);
     QCU_HIP_TRY(hipGetLastError());
-    const C32 *m0 = mcol_ + 1024;
+    const C32 *m0 = mc + 1024;
     return dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
         constexpr int NC = decltype(ncv)::value;
 #define QCU_G8(CS, CO) hipLaunchKernelGGL((k_gather8<CS, CO, NC>), grid, block, 0, s, static_cast<CO *>(out), static_cast<const CS *>(in), q, g32, tab.nbr, m0, g_.Vh, R_)
@@ -598,7 +626,11 @@ hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, dou
         for (int r = 0; r < R_; ++r) L += hbuf_[2 * R_ + r] * hbuf_[r];
         *loss_host = L;
     }
-    // w_k = T_k d on the even sites (the adjoint of hop k is hop k ^ 1), into q
+    return accumulate_grad(in, dlt_, s);
+}
+
+hipError_t LgnnPreconditioner::accumulate_grad(const void *w, const void *dphi, hipStream_t s, const void *hops) {
+    // a_k = T_k dphi on the even sites (the adjoint of hop k is hop k ^ 1), into q
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
     const int spb = R_ >= 256 ? 1 : 256 / R_;
@@ -606,17 +638,17 @@ hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, dou
     QCU_HIP_TRY(dispatch_nc(Nc_, [&](auto ncv) -> hipError_t {
         constexpr int NC = decltype(ncv)::value;
         hipLaunchKernelGGL((k_transport8<Cx<float>, NC>), grid, block, 0, s, static_cast<C32 *>(q_),
-                           static_cast<const C32 *>(dlt_), static_cast<const C32 *>(gauge32_), tab.nbr, g_.Vh, R_, 0);
+                           static_cast<const C32 *>(dphi), static_cast<const C32 *>(gauge32_), tab.nbr, g_.Vh, R_, 0);
         return hipGetLastError();
     }));
     const long ncol = long(g_.Vh) * Nc_ * R_;
     const long fsize = n_ * R_;
     hipLaunchKernelGGL(k_gram, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(q_),
-                       static_cast<const C32 *>(u_), g_.Vh, Nc_ * R_, fsize);
+                       static_cast<const C32 *>(hops ? hops : u_), g_.Vh, Nc_ * R_, fsize);
     QCU_HIP_TRY(hipGetLastError());
     // the input itself is needed in FP32 for the M0 gradient
-    const C32 *inf = static_cast<const C32 *>(in);
-    hipLaunchKernelGGL(k_gram0, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(dlt_), inf, ncol,
+    const C32 *inf = static_cast<const C32 *>(w);
+    hipLaunchKernelGGL(k_gram0, dim3(gram_blocks_), dim3(256), 0, s, gm_, static_cast<const C32 *>(dphi), inf, ncol,
                        Nc_, R_);
     QCU_HIP_TRY(hipGetLastError());
     // partial sums land after the per-block partials
@@ -625,6 +657,102 @@ hipError_t LgnnPreconditioner::loss_grad(const void *in, const void *target, dou
     QCU_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_chain, dim3(1), dim3(256), 0, s, g1_, g2_, w1_, w2_, gsum, C_);
     return hipGetLastError();
+}
+
+hipError_t LgnnPreconditioner::schur32(void *out, const void *in, double kappa, int dagger, hipStream_t s) {
+    DslashArgs a;
+    a.out = ptmp_;
+    a.in = in;
+    a.gauge = gauge32_;
+    a.xpay = nullptr;
+    a.coef = 1.0;
+    a.g = g_;
+    a.Nc = Nc_;
+    a.R = R_;
+    a.parity = 0;
+    a.dagger = dagger;
+    a.prec = kSingle;
+    QCU_HIP_TRY(launch_dslash(a, s));
+    a.out = out;
+    a.in = ptmp_;
+    a.xpay = in;
+    a.coef = -kappa * kappa;
+    a.parity = 1;
+    return launch_dslash(a, s);
+}
+
+hipError_t LgnnPreconditioner::ensure_power(int K, hipStream_t s) {
+    hipError_t err = hipSuccess;
+    const size_t half32 = size_t(n_) * R_ * sizeof(C32);
+    while (int(pw_.size()) < K + 1) {
+        pw_.push_back(dmal(half32, err));
+        if (err != hipSuccess) return err;
+    }
+    while (int(phops_.size()) < K) {  // the 8 hop fields of each factor's input, kept for the gradient
+        phops_.push_back(dmal(8 * half32, err));
+        if (err != hipSuccess) return err;
+    }
+    if (pg_) return hipSuccess;
+    pg_ = dmal(half32, err);
+    pu_ = dmal(half32, err);
+    pn_ = dmal(half32, err);
+    ptmp_ = dmal(half32, err);
+    // [0, R): per-RHS scale, [R, 2R): -1, [2R, 4R): complex ones for the axpy kernels
+    pscale_ = static_cast<double *>(dmal(sizeof(double) * 4 * R_, err));
+    if (err != hipSuccess) return err;
+    for (int r = 0; r < R_; ++r) {
+        hbuf_[r] = -1.0;
+        hbuf_[R_ + 2 * r] = 1.0;
+        hbuf_[R_ + 2 * r + 1] = 0.0;
+    }
+    QCU_HIP_TRY(hipMemcpyAsync(pscale_ + R_, hbuf_, sizeof(double) * 3 * R_, hipMemcpyHostToDevice, s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t LgnnPreconditioner::power_loss_grad(const void *v, int K, double kappa, double *loss_host,
+                                               hipStream_t s) {
+    if (!ready() || K < 1 || K > 8) return hipErrorInvalidValue;
+    QCU_HIP_TRY(ensure_fields(s));
+    QCU_HIP_TRY(ensure_work32(s));
+    QCU_HIP_TRY(ensure_power(K, s));
+    const long N = n_ * R_;
+    const size_t half32 = size_t(N) * sizeof(C32);
+    const C64 *ones = reinterpret_cast<const C64 *>(pscale_ + 2 * R_);
+    // w_0 = v / ||v|| per RHS
+    QCU_HIP_TRY(launch_norm(dnorm_, v, n_, R_, kSingle, true, red_, 0, s));
+    QCU_HIP_TRY(hipMemcpyAsync(hbuf_, dnorm_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
+    QCU_HIP_TRY(hipStreamSynchronize(s));
+    for (int r = 0; r < R_; ++r) hbuf_[r] = hbuf_[r] > 0 ? 1.0 / hbuf_[r] : 0.0;
+    QCU_HIP_TRY(hipMemcpyAsync(pscale_, hbuf_, sizeof(double) * R_, hipMemcpyHostToDevice, s));
+    QCU_HIP_TRY(launch_convert(pw_[0], kSingle, v, kSingle, N, pscale_, R_, s));
+    // w_{j+1} = w_j - A N w_j
+    for (int j = 0; j < K; ++j) {
+        QCU_HIP_TRY(forward(pw_[j], kSingle, phi_, kSingle, s, nullptr, phops_[j]));
+        QCU_HIP_TRY(schur32(pu_, phi_, kappa, 0, s));
+        AxpyArgs ax{kXsay, pw_[j + 1], pw_[j], pu_, nullptr, ones, nullptr, nullptr, n_, R_, kSingle};
+        QCU_HIP_TRY(launch_axpy(ax, s));
+    }
+    if (loss_host) {
+        QCU_HIP_TRY(launch_norm(dloss_, pw_[K], n_, R_, kSingle, false, red_, 1, s));
+        QCU_HIP_TRY(hipMemcpyAsync(hbuf_ + 2 * R_, dloss_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
+        QCU_HIP_TRY(hipStreamSynchronize(s));
+        double L = 0.0;
+        for (int r = 0; r < R_; ++r) L += hbuf_[2 * R_ + r];
+        *loss_host = L;
+    }
+    // back-propagation: g_K = w_K; dL/d(N w_j)^* = -A^+ g_{j+1}; g_j = g_{j+1} - N^+ A^+ g_{j+1}
+    QCU_HIP_TRY(hipMemcpyAsync(pg_, pw_[K], half32, hipMemcpyDeviceToDevice, s));
+    for (int j = K - 1; j >= 0; --j) {
+        QCU_HIP_TRY(schur32(pu_, pg_, kappa, 1, s));
+        if (j > 0) QCU_HIP_TRY(forward(pu_, kSingle, pn_, kSingle, s, madj_));
+        QCU_HIP_TRY(launch_convert(dlt_, kSingle, pu_, kSingle, N, pscale_ + R_, R_, s));
+        QCU_HIP_TRY(accumulate_grad(pw_[j], dlt_, s, phops_[j]));
+        if (j > 0) {
+            AxpyArgs ax{kXsay, pg_, pg_, pn_, nullptr, ones, nullptr, nullptr, n_, R_, kSingle};
+            QCU_HIP_TRY(launch_axpy(ax, s));
+        }
+    }
+    return hipSuccess;
 }
 
 hipError_t LgnnPreconditioner::read_grad(double *gw1, double *gw2, hipStream_t s) {

@@ -69,6 +69,7 @@ struct Context {
     };
     std::vector<void *> lgnn_gauges;  // FP64 copies, one per collected configuration
     std::vector<LgnnBatch> lgnn_batches;
+    int lgnn_power = 0;  // training objective: 0 = (error, residual) pairs, K >= 1 = residual power K
     hipStream_t stream = nullptr;
 
     void clear_lgnn_data() {
@@ -537,7 +538,8 @@ int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *
                   "scatter") ? 0 : g_err;
 }
 int qcuDslashTune(int variant, int by, int bz, int spw) {
-    if (variant < kDslashAuto || variant > kDslashMfma || by < 0 || bz < 0 || spw < 0) {
+    if (variant < kDslashAuto || variant > kDslashMfma || by < 0 || bz < 0 || spw < -64 ||
+        (spw < 0 && variant != kDslashMfma)) {
         set_error(kQcuErrorInvalidValue, "qcuDslashTune: invalid arguments");
         return g_err;
     }
@@ -613,7 +615,8 @@ int qcuDslashTuneLoad(const char *path, int *n_loaded) {
 
 int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
                       int spw) {
-    if (!lattice || variant < kDslashLaneRhs || variant > kDslashMfma || by < 0 || bz < 0 || spw < 0) {
+    if (!lattice || variant < kDslashLaneRhs || variant > kDslashMfma || by < 0 || bz < 0 || spw < -64 ||
+        (spw < 0 && variant != kDslashMfma)) {
         set_error(kQcuErrorInvalidValue, "qcuDslashSetTuned: invalid arguments");
         return g_err;
     }
@@ -706,6 +709,12 @@ int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, 
         return kQcuErrorNotConverged;
     }
     return 0;
+}
+
+int qcuGroupsDslashVariant(void) {
+    Context *c = ctx_or_error();
+    if (!c) return -1;
+    return c->groups ? c->groups->dslash_variant() : 0;
 }
 
 // ---- learned preconditioner --------------------------------------------
@@ -852,6 +861,7 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
         set_error(kQcuErrorNotInitialized, "qcuLgnnTrain: needs weights (qcuLgnnSetWeights) and collected data");
         return g_err;
     }
+    const long nR = c->half_n() * c->R;
     std::vector<size_t> order(c->lgnn_batches.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
@@ -876,7 +886,17 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
             }
             double L = 0.0;
             ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn train");
-            ok = ok && hip_ok(c->precond->loss_grad(bt.r, bt.e, &L, c->stream), "lgnn train");
+            if (c->lgnn_power > 0) {
+                // a fixed complex Gaussian vector per batch (the batch index is its key)
+                const uint64_t key = (uint64_t(seed) << 32) ^ (uint64_t(order[i]) * 0x9E3779B97F4A7C15ull);
+                ok = ok && hip_ok(launch_gaussian(static_cast<Cx<double> *>(c->bufA), nR, key, c->stream), "lgnn train");
+                ok = ok && hip_ok(launch_convert(c->bufB, kSingle, c->bufA, kDouble, nR, nullptr, c->R, c->stream),
+                                  "lgnn train");
+                ok = ok && hip_ok(c->precond->power_loss_grad(c->bufB, c->lgnn_power, c->kappa, &L, c->stream),
+                                  "lgnn train");
+            } else {
+                ok = ok && hip_ok(c->precond->loss_grad(bt.r, bt.e, &L, c->stream), "lgnn train");
+            }
             ok = ok && hip_ok(c->precond->adam_step(lr, 1, c->stream), "lgnn train");
             total += L;
         }
@@ -884,6 +904,32 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
     }
     ok = ok && hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
     ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn train");
+    return ok ? 0 : g_err;
+}
+
+int qcuLgnnSetObjective(int power) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (power < 0 || power > 8) {
+        set_error(kQcuErrorInvalidValue, "qcuLgnnSetObjective: power must be 0 (pair loss) or 1..8");
+        return g_err;
+    }
+    c->lgnn_power = power;
+    return 0;
+}
+
+int qcuLgnnPowerLossGrad(const void *v, int power, double *loss, double *gw1, double *gw2) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (!c->precond || !c->precond->ready() || !v || power < 1 || power > 8 || !c->have_dslash) {
+        set_error(kQcuErrorInvalidValue, "qcuLgnnPowerLossGrad: needs weights, getDslash and 1 <= power <= 8");
+        return g_err;
+    }
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn power");
+    ok = ok && hip_ok(c->precond->power_loss_grad(v, power, c->kappa, loss, c->stream), "lgnn power");
+    if (ok && gw1 && gw2) ok = hip_ok(c->precond->read_grad(gw1, gw2, c->stream), "lgnn power");
+    ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn power");
     return ok ? 0 : g_err;
 }
 

@@ -194,4 +194,16 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 
 inline int div_up(long a, long b) { return int((a + b - 1) / b); }
 
+// compute units of the current device (queried once; 256 on MI355X)
+inline int num_cus() {
+    static const int n = [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        return v;
+    }();
+    return n;
+}
+
 }  // namespace qcu

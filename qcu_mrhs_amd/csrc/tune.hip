@@ -94,9 +94,12 @@ hipError_t dslash_autotune(const int L[4], int Nc, int R, int prec, int reps, Ds
     divisors_upto(L[2], 12, bzs);
     const int area0 = std::max(1, def.by * def.bz);
     std::vector<int> variants = {kDslashWaveSite};
-    if (dslash_mfma_supported(Nc, prec)) variants.push_back(kDslashMfma);
+    if (dslash_matrix_supported(Nc, prec)) variants.push_back(kDslashMfma);
     for (int v : variants) {
-        const std::vector<int> spws = v == kDslashMfma ? std::vector<int>{1, 2, 4} : std::vector<int>{1, 2, 4, 8};
+        // FP16/BF16 MFMA: also the persistent grid of 4, 5, 6 or 8 workgroups per CU (spw < 0)
+        const std::vector<int> spws = v != kDslashMfma ? std::vector<int>{1, 2, 4, 8}
+                                      : dslash_mfma_supported(Nc, prec) ? std::vector<int>{1, 2, 4, -4, -5, -6, -8}
+                                                                        : std::vector<int>{1, 2, 4};
         for (int spw : spws) {
             // traversal tiles: square-ish (y, z) tiles from a quarter to four times the default
             // tile area, and the untiled order
@@ -192,7 +195,8 @@ hipError_t dslash_tune_load(const char *path, int &n_loaded) {
         int v[11];
         if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
                    &v[7], &v[8], &v[9], &v[10]) != 11 ||
-            v[7] < kDslashLaneRhs || v[7] > kDslashMfma || v[8] < 0 || v[9] < 0 || v[10] < 0 ||
+            v[7] < kDslashLaneRhs || v[7] > kDslashMfma || v[8] < 0 || v[9] < 0 || v[10] < -64 ||
+            (v[10] < 0 && v[7] != kDslashMfma) ||
             !dslash_supported(v[4], v[5])) {
             err = hipErrorInvalidValue;
             break;

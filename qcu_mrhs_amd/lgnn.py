@@ -118,3 +118,22 @@ def train(epochs, learning_rate, seed=0):
 
 def clear_data():
     api.check(api.lib().qcuLgnnClearData())
+
+
+def set_objective(power):
+    """Training objective of train(): 0 = pair loss sum ||N r - e||^2 / ||e||^2
+    on the collected (error, residual) pairs; K = 1..8 = residual-power loss
+    sum ||(1 - A N)^K v||^2 on a fixed unit Gaussian vector per stored batch,
+    which weights the modes that set the preconditioned iteration count."""
+    api.check(api.lib().qcuLgnnSetObjective(int(power)))
+
+
+def power_loss_grad(v, power, channels):
+    """(loss, dL/dRe W1 + i dL/dIm W1, same for W2) of the residual-power loss
+    sum_r ||(1 - A N)^power v_r / ||v_r||||^2 for one batch of FP32 odd-parity
+    multi-RHS vectors (device buffer) on the loaded gauge field."""
+    g1 = np.zeros((channels, NHOP, 4, 4), dtype=np.complex128)
+    g2 = np.zeros_like(g1)
+    loss = ctypes.c_double(0.0)
+    api.check(api.lib().qcuLgnnPowerLossGrad(api._p(v), int(power), ctypes.byref(loss), _dptr(g1), _dptr(g2)))
+    return loss.value, g1, g2

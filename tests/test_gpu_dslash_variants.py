@@ -52,10 +52,12 @@ def test_variant_matches_oracle(variant, L, Nc, R):
 
 
 @pytest.mark.parametrize("variant", ["wave", "mfma"])
-@pytest.mark.parametrize("by,bz,spw", [(1, 1, 1), (2, 4, 3), (4, 2, 5), (8, 8, 2)])
+@pytest.mark.parametrize("by,bz,spw", [(1, 1, 1), (2, 4, 3), (4, 2, 5), (8, 8, 2), (2, 4, -1), (8, 8, -5)])
 def test_traversal_tiles_and_sites_per_wave(variant, by, bz, spw):
     """Every (y, z) tiling and sites-per-wave count covers each site exactly once."""
     L, Nc, R = (6, 8, 8, 6), 3, 8
+    if spw < 0 and variant != "mfma":
+        pytest.skip("the persistent grid (spw < 0) exists for the MFMA kernel only")
     prec = H if variant == "mfma" else D
     gauge = fields.gauge_field(L, Nc, 7, kind="haar")
     inp = fields.to_mrhs(fields.gaussian_sources(L, Nc, R, 8).reshape(R, 2, -1)[:, 0], Nc)
@@ -69,3 +71,55 @@ def test_traversal_tiles_and_sites_per_wave(variant, by, bz, spw):
     got = d_out.complex(inp.size, prec)
     per_site = np.abs(got - ref).reshape(-1, 4 * Nc * R).max(axis=1) / np.abs(ref).max()
     assert per_site.max() < TOL[prec] * 10, np.argmax(per_site)
+
+
+@pytest.mark.parametrize("prec", [H, B])
+@pytest.mark.parametrize("spw", [-4, -8])
+def test_mfma_persistent_grid_production_shape(prec, spw):
+    """Persistent MFMA grid (spw < 0: a fixed number of workgroups per CU
+    striding through their XCD's share of the sites) at 16^3 x 32 with two
+    RHS chunks (R = 20), both parities and the xpay form, against the oracle."""
+    L, Nc, R = (16, 16, 16, 32), 3, 20
+    print("case persistent", prec, spw, flush=True)
+    gauge = fields.gauge_field(L, Nc, 61, kind="smooth", eps=0.3)
+    src = fields.gaussian_sources(L, Nc, R, 62).reshape(R, 2, -1)
+    api.dslash_tune("mfma", 6, 8, spw)
+    d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+    for parity, dagger in ((1, 0), (0, 1)):
+        inp = fields.to_mrhs(src[:, 1 - parity], Nc)
+        xp = fields.to_mrhs(src[:, parity], Nc)
+        d_in = api.DeviceArray.from_complex(inp, prec)
+        d_xp = api.DeviceArray.from_complex(xp, prec)
+        d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+        ref = oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)
+        api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec)
+        assert _rel(d_out.complex(inp.size, prec), ref) < TOL[prec]
+        api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
+        assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec]
+# to append to tests/test_gpu_dslash_variants.py
+
+
+@pytest.mark.parametrize("L,Nc,R", [((4, 4, 4, 4), 8, 18), ((4, 4, 4, 8), 4, 5), ((6, 4, 4, 4), 3, 17),
+                                    ((4, 4, 4, 8), 2, 3), ((16, 16, 16, 16), 8, 16)])
+def test_fp32_matrix_core_dslash_matches_oracle(L, Nc, R):
+    """FP32 storage on the FP32 matrix cores (v_mfma_f32_16x16x4_f32): both
+    parities, dagger, the xpay form, one and several sites per wavefront,
+    RHS counts with partial 16-RHS chunks, against the FP64 oracle at the
+    FP32 storage bound."""
+    print("case mfma32", L, Nc, R, flush=True)
+    gauge = fields.gauge_field(L, Nc, 500 + Nc + R, kind="haar")
+    src = fields.gaussian_sources(L, Nc, R, 600 + R).reshape(R, 2, -1)
+    d_g = api.DeviceArray.from_complex(gauge.reshape(-1), S)
+    for spw in (1, 4):
+        api.dslash_tune("mfma", 0, 0, spw)
+        for parity, dagger in ((1, 0), (0, 1)):
+            inp = fields.to_mrhs(src[:, 1 - parity], Nc)
+            xp = fields.to_mrhs(src[:, parity], Nc)
+            d_in = api.DeviceArray.from_complex(inp, S)
+            d_xp = api.DeviceArray.from_complex(xp, S)
+            d_out = api.DeviceArray(api.storage_bytes(inp.size, S))
+            ref = oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=S)
+            assert _rel(d_out.complex(inp.size, S), ref) < TOL[S], (spw, parity, dagger)
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=S, xpay=d_xp, coef=-0.13)
+            assert _rel(d_out.complex(inp.size, S), xp - 0.13 * ref) < TOL[S], (spw, "xpay")

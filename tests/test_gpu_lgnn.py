@@ -141,9 +141,37 @@ def test_loss_and_gradients_match_oracle():
     del d_g
 
 
-def _solve(Lt, gauge, b, kappa, use_pc, inner=2, tol=1e-10):
+@pytest.mark.parametrize("K", [1, 3])
+def test_power_loss_and_gradients_match_oracle(K):
+    """Residual-power loss ||(1 - A N)^K v / ||v||||^2 (forward and back-propagation
+    through A, A^+, N, N^+ on the GPU in FP32) against the oracle's layer-form
+    back-propagation with the FP64 Schur operator."""
+    _case("lgnn power gradient", K)
+    gauge = fields.gauge_field(L, Nc, 95, kind="smooth", eps=0.4)
+    d_g = _setup(L, Nc, R, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    v = _odd_field(96, (Vh, 4, Nc, R)).astype(np.complex64).astype(np.complex128)
+    net = olgnn.Net.random(8, 97, scale=0.15)
+    lgnn.set_weights(*_abi_weights(net))
+    d_v = api.DeviceArray.from_complex(v.reshape(-1), 1)
+    loss, g1, g2 = lgnn.power_loss_grad(d_v, K, 8)
+
+    def schur(x, dag):
+        return oracle.schur(x.reshape(-1), gauge, L, Nc, R, KAPPA, dag)
+
+    ref_loss, ref_g1, ref_g2 = net.power_loss_and_grad(olgnn.Transport(gauge, L, Nc), schur, v, K, L, Nc, R)
+    print(f"K {K}: loss {loss:.8e} (oracle {ref_loss:.8e}); gradient errors "
+          f"{_rel(g1, ref_g1[:, 0]):.2e}, {_rel(g2, ref_g2[0]):.2e}")
+    assert abs(loss - ref_loss) <= 1e-5 * abs(ref_loss)
+    assert _rel(g1, ref_g1[:, 0]) < RTOL_GRAD
+    assert _rel(g2, ref_g2[0]) < RTOL_GRAD
+    api.finalizeQcu()
+    del d_g
+
+
+def _solve(b, use_pc, inner=2, tol=1e-10, delta=0.0):
     R_ = b.shape[0]
-    api.set_solver_options(inner, 0.1, use_pc)
+    api.set_solver_options(inner, delta, use_pc)
     bs = [api.DeviceArray.from_complex(b[k]) for k in range(R_)]
     xs = [api.DeviceArray(b[k].nbytes) for k in range(R_)]
     for k in range(R_):
@@ -153,64 +181,98 @@ def _solve(Lt, gauge, b, kappa, use_pc, inner=2, tol=1e-10):
     return x, info
 
 
-def test_training_lowers_loss_and_iterations_on_held_out_config():
-    """8^4 SU(3), exp(i 0.3 H) links, kappa 0.135, R = 4: pairs from plain
-    FP64 solves on 4 configurations (16 iterate snapshots and 8 Gaussian
-    errors per RHS and configuration), Adam on the GPU, then the trained network
-    as a right preconditioner on a held-out configuration.  Checked: the
-    training loss falls, the preconditioned solve reaches a FP64 true residual
-    of 1e-10 (through the oracle operator) and takes fewer iterations than
-    the solve without the network (the untrained Neumann start is printed
-    beside it)."""
-    Lt, kappa, Rt, C = (8, 8, 8, 8), 0.135, 4, 8
-    _case("lgnn train", Lt)
-    gauges = [fields.gauge_field(Lt, Nc, 500 + k, kind="smooth", eps=0.3) for k in range(5)]
+def _train(Lt, Rt, eps, kappa, n_cfg, epochs=12, lr=2e-3, power=3):
+    """Batches from plain FP64 solves of n_cfg seeded configurations (their
+    gauge fields; 16 iterate snapshots + 16 Gaussian errors per RHS for the
+    pair loss), training from the first Neumann term with the residual-power
+    loss on the GPU.  Returns (held-out gauge, held-out sources, device gauge,
+    Neumann start weights, trained weights, losses)."""
+    gauges = [fields.gauge_field(Lt, Nc, 7000 + k, kind="smooth", eps=eps) for k in range(n_cfg)]
     d_g = _setup(Lt, Nc, Rt, gauges[0], kappa)
-    w1, w2 = lgnn.neumann_weights(kappa, channels=C, noise=1e-3, seed=5)
+    w1, w2 = lgnn.neumann_weights(kappa, channels=8, noise=1e-3, seed=11)
     lgnn.set_weights(w1, w2)
     lgnn.clear_data()
-    nb = 0
-    for k in range(4):
+    for k in range(n_cfg):
         d_g.upload(gauges[k].reshape(-1))
         api.loadQcuGauge(d_g, 2)
-        b = fields.gaussian_sources(Lt, Nc, Rt, 600 + k)
+        b = fields.gaussian_sources(Lt, Nc, Rt, 7100 + k)
         bs = [api.DeviceArray.from_complex(b[j]) for j in range(Rt)]
         xs = [api.DeviceArray(b[j].nbytes) for j in range(Rt)]
         for j in range(Rt):
             api.pushBackFermions(xs[j], bs[j])
-        nb = lgnn.collect(5000, 1e-10, 16, n_random=8, seed=k)
-    assert nb == 96, nb
-    losses = lgnn.train(40, 2e-3, seed=3)
-    print("training loss per epoch:", " ".join(f"{v:.4f}" for v in losses))
-    assert np.all(np.isfinite(losses))
-    assert losses[-1] < losses[0]
-    trained = lgnn.get_weights(C)
+        lgnn.collect(5000, 1e-10, 16, n_random=16, seed=k)
+    lgnn.set_objective(power)
+    losses = lgnn.train(epochs, lr, seed=5)
+    lgnn.set_objective(0)
+    trained = lgnn.get_weights(8)
     lgnn.clear_data()
-
-    # held-out configuration
-    d_g.upload(gauges[4].reshape(-1))
+    held = fields.gauge_field(Lt, Nc, 7900, kind="smooth", eps=eps)
+    d_g.upload(held.reshape(-1))
     api.loadQcuGauge(d_g, 2)
-    b = fields.gaussian_sources(Lt, Nc, Rt, 700)
-    _, plain = _solve(Lt, gauges[4], b, kappa, False)
-    lgnn.set_weights(w1, w2)
-    _, init = _solve(Lt, gauges[4], b, kappa, True)
-    lgnn.set_weights(*trained)
-    x, pc = _solve(Lt, gauges[4], b, kappa, True)
+    b = fields.gaussian_sources(Lt, Nc, Rt, 7901)
+    return held, b, d_g, lgnn.neumann_weights(kappa, channels=8), trained, losses
+
+
+def _true_rel(x, b, gauge, Lt, Rt, kappa):
     bm = fields.to_mrhs(b, Nc)
-    r = bm - oracle.mat(fields.to_mrhs(x, Nc), gauges[4], Lt, Nc, Rt, kappa)
-    true_rel = (oracle.norm(r, Rt) / oracle.norm(bm, Rt)).max()
-    print(f"held-out iterations: plain {plain['iterations']}, Neumann init {init['iterations']}, "
-          f"trained {pc['iterations']}; trained true residual {true_rel:.2e}")
-    assert plain["converged"] and init["converged"] and pc["converged"]
-    assert init["preconditioned"] == 1 and pc["preconditioned"] == 1 and pc["fallback"] == 0
-    assert true_rel < 1.5e-10 and pc["true_rel_residual"] < 1.5e-10
-    # the gain over the solver without the network; the Neumann start is reported beside it
-    assert pc["iterations"] < plain["iterations"]
-    # mixed precision (FP16 inner) with the trained network converges to the same bar
-    x16, pc16 = _solve(Lt, gauges[4], b, kappa, True, inner=0)
-    r = bm - oracle.mat(fields.to_mrhs(x16, Nc), gauges[4], Lt, Nc, Rt, kappa)
-    true16 = (oracle.norm(r, Rt) / oracle.norm(bm, Rt)).max()
-    print(f"FP16 inner + trained network: {pc16['iterations']} iterations, "
-          f"{pc16['reliable_updates']} reliable updates, true residual {true16:.2e}")
-    assert pc16["converged"] and true16 < 1.5e-10
+    r = bm - oracle.mat(fields.to_mrhs(x, Nc), gauge, Lt, Nc, Rt, kappa)
+    return (oracle.norm(r, Rt) / oracle.norm(bm, Rt)).max()
+
+
+def test_training_beats_the_untrained_start_on_held_out_config():
+    """8^4 SU(3), exp(i 0.3 H) links, kappa 0.135, R = 4, 4 training
+    configurations, residual-power loss (K = 3): on a held-out configuration
+    the trained network as right preconditioner takes fewer iterations than
+    its untrained start (the first Neumann term) and than the solver without
+    it, in plain FP64 and with FP16 inner precision, each solve reaching a
+    FP64 true residual of 1e-10 through the oracle operator."""
+    Lt, Rt, kappa = (8, 8, 8, 8), 4, 0.135
+    _case("lgnn train", Lt)
+    held, b, d_g, neumann, trained, losses = _train(Lt, Rt, 0.3, kappa, 4)
+    print("training loss per epoch:", " ".join(f"{v:.3e}" for v in losses))
+    assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]
+    res = {}
+    for inner in (2, 0):
+        _, plain = _solve(b, False, inner)
+        lgnn.set_weights(*neumann)
+        _, init = _solve(b, True, inner)
+        lgnn.set_weights(*trained)
+        x, pc = _solve(b, True, inner)
+        tr = _true_rel(x, b, held, Lt, Rt, kappa)
+        res[inner] = (plain, init, pc)
+        print(f"inner {inner}: iterations plain {plain['iterations']}, Neumann start {init['iterations']}, "
+              f"trained {pc['iterations']} ({pc['reliable_updates']} reliable updates); times "
+              f"{plain['seconds'] * 1e3:.2f} / {init['seconds'] * 1e3:.2f} / {pc['seconds'] * 1e3:.2f} ms; "
+              f"trained true residual {tr:.2e}")
+        assert plain["converged"] and init["converged"] and pc["converged"]
+        assert init["preconditioned"] == 1 and pc["preconditioned"] == 1 and pc["fallback"] == 0
+        assert tr < 1.5e-10 and pc["true_rel_residual"] < 1.5e-10
+        assert pc["iterations"] < init["iterations"] < plain["iterations"]
     api.finalizeQcu()
+    del d_g
+
+
+@pytest.mark.parametrize("kappa", [0.130, 0.1345, 0.1355])
+def test_training_at_production_volume_and_kappa(kappa):
+    """16^3 x 32 SU(3), exp(i 0.5 H) links, 16 RHS, kappa 0.130 / 0.1345 /
+    0.1355, 4 training configurations, residual-power loss (K = 3): on a
+    held-out configuration the trained network needs no more iterations than
+    its untrained start and fewer than the solver without it (FP16 inner,
+    FP64 reliable updates, FP64 true residual 1e-10 through the oracle)."""
+    Lt, Rt = (16, 16, 16, 32), 16
+    _case("lgnn train production", kappa)
+    held, b, d_g, neumann, trained, losses = _train(Lt, Rt, 0.5, kappa, 4)
+    assert np.all(np.isfinite(losses)) and losses[-1] <= losses[0]
+    _, plain = _solve(b, False, 0)
+    lgnn.set_weights(*neumann)
+    _, init = _solve(b, True, 0)
+    lgnn.set_weights(*trained)
+    x, pc = _solve(b, True, 0)
+    tr = _true_rel(x, b, held, Lt, Rt, kappa)
+    print(f"kappa {kappa}: FP16 inner iterations plain {plain['iterations']}, Neumann start "
+          f"{init['iterations']}, trained {pc['iterations']}; times {plain['seconds'] * 1e3:.1f} / "
+          f"{init['seconds'] * 1e3:.1f} / {pc['seconds'] * 1e3:.1f} ms; trained true residual {tr:.2e}")
+    assert pc["converged"] and pc["fallback"] == 0 and tr < 1.5e-10
+    assert pc["iterations"] <= init["iterations"] < plain["iterations"]
+    api.finalizeQcu()
+    del d_g

@@ -102,3 +102,53 @@ def test_gradients_match_finite_differences():
                 fd = (lp - lm) / (2 * eps)
                 an = G[idx].real if part == "re" else G[idx].imag
                 assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (which, idx, part, fd, an)
+
+
+def test_adjoint_network_is_the_adjoint():
+    """<y, N x> = <N^+ y, x> for random odd fields (pins adjoint_apply)."""
+    gauge, tr, x = _setup(8)
+    g = np.random.default_rng(3)
+    y = g.standard_normal(x.shape) + 1j * g.standard_normal(x.shape)
+    net = lgnn.Net.random(3, 12, scale=0.3)
+    nx, _ = net.forward(tr, x, L, Nc, R)
+    nty = net.adjoint_apply(tr, y, L, Nc, R)
+    lhs, rhs = np.vdot(y, nx), np.vdot(nty, x)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("K", [1, 3])
+def test_power_loss_gradients_match_finite_differences(K):
+    """Residual-power loss ||(1 - A N)^K v / ||v||||^2 with A the dslash
+    oracle's odd Schur operator: the back-propagated gradient equals central
+    finite differences of the loss; with the Neumann weights and K = 1 the
+    loss is ||kappa^4 (D_oe D_eo)^2 w||^2 (1 - A (1 + K) = K^2, K = kappa^2 D_oe D_eo)."""
+    kappa = 0.12
+    gauge, tr, v = _setup(9, kind="smooth")
+
+    def schur(x, dag):
+        return oracle.schur(x.reshape(-1), gauge, L, Nc, R, kappa, dag)
+
+    if K == 1:
+        net = _neumann_net(kappa)
+        loss, _, _ = net.power_loss_and_grad(tr, schur, v, 1, L, Nc, R)
+        w = (v / np.sqrt((np.abs(v) ** 2).sum(axis=(0, 1, 2)))).reshape(-1)
+        kw = w - schur(w, 0)
+        k2w = kw - schur(kw, 0)
+        assert abs(loss - np.vdot(k2w, k2w).real) <= 1e-10 * loss
+    net = lgnn.Net.random(2, 13, scale=0.2)
+    loss, gW1, gW2 = net.power_loss_and_grad(tr, schur, v, K, L, Nc, R)
+    g = np.random.default_rng(4)
+    eps = 1e-6
+    for which, W, G in (("W1", net.W1, gW1), ("W2", net.W2, gW2)):
+        for _ in range(4):
+            idx = tuple(g.integers(0, n) for n in W.shape)
+            for dz in (1.0, 1j):
+                Wp, Wm = W.copy(), W.copy()
+                Wp[idx] += eps * dz
+                Wm[idx] -= eps * dz
+                netp = lgnn.Net(Wp, net.W2) if which == "W1" else lgnn.Net(net.W1, Wp)
+                netm = lgnn.Net(Wm, net.W2) if which == "W1" else lgnn.Net(net.W1, Wm)
+                fd = (netp.power_loss_and_grad(tr, schur, v, K, L, Nc, R)[0] -
+                      netm.power_loss_and_grad(tr, schur, v, K, L, Nc, R)[0]) / (2 * eps)
+                an = G[idx].real if dz == 1.0 else G[idx].imag
+                assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (which, idx, dz, fd, an)
