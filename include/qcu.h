@@ -77,14 +77,22 @@ void pushBackFermions(void *fermionOut, void *fermionIn);
 /* src/interface.cu:45-48 / src/qcu_interface.cu:257-268: copy an even/odd
  * gauge field [mu][parity][t][z][y][x/2][Nc][Nc] into internal storage. */
 void loadQcuGauge(void *gauge, int floatPrecision);
-/* src/interface.cu:50-53 / src/qcu_interface.cu:66-107: kappa = 1/(2(4+mass)). */
+/* src/interface.cu:50-53 / src/qcu_interface.cu:66-107: kappa = 1/(2(4+mass)).
+ * dslashType kQcuDslashWilson: M = 1 - kappa D.  kQcuDslashClover (declared
+ * by the reference, include/qcu_helper_macro.h:7-11, but rejected by its
+ * getDslash, src/qcu_interface.cu:97-105): the Wilson-clover operator
+ * M = A - kappa D, A = 1 - kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu}
+ * with the clover-leaf field strength; c_sw from qcuSetCloverCoefficient
+ * (default 1).  The term and its inverse are built on the device at the
+ * first mat_Qcu / qcuInvert after the gauge field, kappa or c_sw changed. */
 void getDslash(int dslashType, double mass);
 /* src/interface.cu:55-58 / src/qcu_interface.cu:109-172: out = D in on one
  * parity; queued pointers are half-lattice vectors, `parity` is the output
  * parity. */
 void start_dslash(int parity, int daggerFlag);
 /* src/interface.cu:59-62 / src/qcu_interface.cu:174-255: out = (1 - kappa D) in
- * on full-lattice vectors. */
+ * on full-lattice vectors ((A - kappa D) in with the clover type).  start_dslash
+ * applies the hopping term D alone for either type. */
 void mat_Qcu(int daggerFlag);
 /* src/interface.cu:70-73 / src/qcu_interface.cu:280-362: solve M x = b for
  * every queued (x, b) pair; even/odd preconditioned BiCGStab. */
@@ -119,6 +127,10 @@ typedef struct QcuSolveInfo_s {
     int fallback;              /* 1 if the preconditioned solve failed and
                                   plain BiCGStab (from x0 = 0) produced x     */
 } QcuSolveInfo;
+
+/* Clover coefficient c_sw of the kQcuDslashClover operator (default 1.0);
+ * takes effect at the next mat_Qcu / qcuInvert.  Requires initGridSize. */
+int qcuSetCloverCoefficient(double csw);
 
 /* qcuInvert with status: returns kQcuSuccess or an error code, fills info. */
 int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info);

@@ -4,6 +4,7 @@
  * and BLAS interfaces that the reference keeps in C++:
  *
  *   qcuDslashMrhs       <- qcu::Dslash::apply(DslashParam) (include/qcd/qcu_dslash.h:10-74)
+ *   qcuCloverBuild/Apply   the clover term the reference declares but does not implement
  *   qcuBlas*            <- qcu::qcu_blas functors (include/qcu_blas/qcu_blas_complex_*.h,
  *                          src/qcu_blas/qcu_complex_{xsay,xpay,axpby,axpbypcz,dotc,norm}.cu)
  *   qcuColorSpinorGather/Scatter <- colorSpinorGather/Scatter
@@ -29,6 +30,22 @@ extern "C" {
  * out = xpay + coef * D in when xpay != NULL (else out = coef * D in). */
 int qcuDslashMrhs(void *out, const void *in, const void *gauge, const int lattice[4], int n_color,
                   int m_rhs, int parity, int dagger, int precision, const void *xpay, double coef);
+
+/* Clover term of the Wilson-clover operator (no reference counterpart: the
+ * reference declares the type, include/qcu_helper_macro.h:7-11, and rejects it,
+ * src/qcu_interface.cu:97-105).  qcuCloverBuild writes
+ * A = 1 - kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu} from an FP64 even/odd
+ * gauge field as FP64 blocks clover[parity 2][Vh][chirality 2][2Nc][2Nc]
+ * (row/column = spin within the pair {0,1} or {2,3} times Nc + colour) and,
+ * when clover_inv != NULL, the inverse of every block in the same layout. */
+int qcuCloverBuild(void *clover, void *clover_inv, const void *gauge, const int lattice[4], int n_color,
+                   double csw, double kappa);
+/* out = A in + coef * xpay (xpay may be NULL) on one parity's MRHS half
+ * vectors [Vh][4][Nc][R] in `precision`; `blocks` = that parity's
+ * [Vh][2][2Nc][2Nc] (a term or its inverse) in clover_precision
+ * (kQcuPrecisionDouble or kQcuPrecisionSingle).  out may equal in or xpay. */
+int qcuCloverApply(void *out, const void *in, const void *blocks, const int lattice[4], int n_color, int m_rhs,
+                   int precision, int clover_precision, const void *xpay, double coef);
 
 /* res = x - a_r y  (src/qcu_blas/qcu_complex_xsay.cu) */
 int qcuBlasXsay(void *res, const void *x, const void *a, const void *y, long n, int m_rhs,

@@ -152,3 +152,88 @@ def transform_fermion(psi_full, gx, Nc):
     V = gx.shape[0]
     v = psi_full.reshape(V, 4, Nc)
     return np.einsum("nab,nsb->nsa", gx, v).reshape(-1)
+
+
+def sigma(mu, nu):
+    """sigma_{mu nu} = (i/2) [gamma_mu, gamma_nu] (4x4)."""
+    return 0.5j * (GAMMA[mu] @ GAMMA[nu] - GAMMA[nu] @ GAMMA[mu])
+
+
+def field_strength(gauge_eo, L, Nc):
+    """F_{mu nu}(x) for mu < nu, as [6][V][Nc][Nc] in full-vector site order:
+    the clover average of the four plaquette leaves, made Hermitian and
+    traceless, F = ((Q - Q^+) - tr(Q - Q^+)/Nc) / (8 i).  Written with whole-
+    lattice coordinate arrays (no per-site loop), independently of the C
+    oracle's path products."""
+    L = tuple(int(v) for v in L)
+    c = coords(L)
+    Lv = np.array(L)
+
+    def U(mu, shift):
+        return link(gauge_eo, L, mu, (c + np.asarray(shift)) % Lv)
+
+    def dag(m):
+        return np.conj(np.swapaxes(m, -1, -2))
+
+    out = []
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            e_mu = np.eye(4, dtype=np.int64)[mu]
+            e_nu = np.eye(4, dtype=np.int64)[nu]
+            z = np.zeros(4, dtype=np.int64)
+            leaves = (
+                U(mu, z) @ U(nu, e_mu) @ dag(U(mu, e_nu)) @ dag(U(nu, z)),
+                U(nu, z) @ dag(U(mu, e_nu - e_mu)) @ dag(U(nu, -e_mu)) @ U(mu, -e_mu),
+                dag(U(mu, -e_mu)) @ dag(U(nu, -e_mu - e_nu)) @ U(mu, -e_mu - e_nu) @ U(nu, -e_nu),
+                dag(U(nu, -e_nu)) @ U(mu, -e_nu) @ U(nu, e_mu - e_nu) @ dag(U(mu, z)),
+            )
+            Q = sum(leaves)
+            A = Q - dag(Q)
+            tr = np.trace(A, axis1=1, axis2=2)
+            A = A - (tr / Nc)[:, None, None] * np.eye(Nc)[None]
+            out.append(A / 8j)
+    return np.stack(out)
+
+
+def clover_site_matrices(gauge_eo, L, Nc, csw, kappa):
+    """A(x) = 1 - kappa c_sw sum_{mu<nu} sigma_{mu nu} (x) F_{mu nu}(x), the full
+    4Nc x 4Nc matrix per site ([V][4Nc][4Nc], rows spin-major), built from the
+    explicit 4x4 gamma matrices."""
+    F = field_strength(gauge_eo, L, Nc)
+    V = F.shape[1]
+    A = np.broadcast_to(np.eye(4 * Nc, dtype=np.complex128), (V, 4 * Nc, 4 * Nc)).copy()
+    q = 0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            s = sigma(mu, nu)
+            A -= kappa * csw * np.einsum("st,nab->nsatb", s, F[q]).reshape(V, 4 * Nc, 4 * Nc)
+            q += 1
+    return A
+
+
+def clover_matrix(gauge_eo, L, Nc, csw, kappa):
+    """Sparse block-diagonal full-lattice clover matrix A."""
+    A = clover_site_matrices(gauge_eo, L, Nc, csw, kappa)
+    return sp.block_diag(list(A), format="csr")
+
+
+def constant_abelian_field(L, Nc, mu, nu, n_flux):
+    """Links of a constant abelian field in the (mu, nu) plane: U_nu(x) =
+    exp(i B x_mu tau), U_mu = 1 except U_mu(x_mu = L_mu - 1) = exp(-i B L_mu x_nu
+    tau), the other links 1, with tau = diag(1, -1, 0, ...) and B = 2 pi n_flux /
+    (L_mu L_nu) (n_flux quanta through the periodic plane).  Every (mu, nu)
+    plaquette U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ equals exp(i B tau) and
+    every other plaquette is 1.  Returns (even/odd gauge [4][2][Vh][Nc][Nc], B)."""
+    L = tuple(int(v) for v in L)
+    V = int(np.prod(L))
+    B = 2 * np.pi * n_flux / (L[mu] * L[nu])
+    c = coords(L)
+    th = np.zeros((4, V))
+    th[nu] = B * c[:, mu]
+    th[mu] = np.where(c[:, mu] == L[mu] - 1, -B * L[mu] * c[:, nu], 0.0)
+    tau = np.zeros(Nc)
+    tau[0], tau[1] = 1.0, -1.0
+    U = np.zeros((4, V, Nc, Nc), dtype=np.complex128)
+    U[:, :, np.arange(Nc), np.arange(Nc)] = np.exp(1j * th[:, :, None] * tau[None, None, :])
+    # coords() lists sites in full-vector order, which is the even/odd storage order
+    return U.reshape(4, 2, V // 2, Nc, Nc), B

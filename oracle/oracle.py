@@ -22,6 +22,15 @@ What pins each oracle function (asserted in tests/test_oracle_*.py):
                  of the same shape as those tests.
 * gather/scatter the reference gather test (tests/test_gather/test_gather.cu:
                  32-85: R=1 gather is the identity) and the round trip.
+* ``clover``     unit links give A = 1; a constant abelian field gives the
+                 closed form 1 - kappa c_sw sigma_{mu nu} sin(B) tau; equality with
+                 the whole-lattice numpy construction from the explicit 4x4
+                 gamma matrices (oracle/dense.py); Hermitian blocks; gauge
+                 covariance; the sign and normalisation of the term by the
+                 O(a) cancellation it exists for (the would-be zero modes of a
+                 constant-field background move to O(a^2) at c_sw = 1).
+* ``clover_apply``, ``clover_invert``  the dense block-diagonal matrix and
+                 numpy.linalg.inv.
 * ``gauge_eo``   parity unpinned: the reference's converter lives in the
                  absent qcu_io submodule; only the round trip and agreement
                  with the site convention of include/point/qcu_point.cuh are
@@ -70,6 +79,9 @@ def lib():
         L.oracle_gather.argtypes = [_c128p, ctypes.POINTER(vp), l, i, i]
         L.oracle_scatter.argtypes = [ctypes.POINTER(vp), _c128p, l, i, i]
         L.oracle_gauge_eo.argtypes = [_c128p, _c128p, _i32p, i, i]
+        L.oracle_clover.argtypes = [_c128p, _c128p, _i32p, i, d, d]
+        L.oracle_clover_apply.argtypes = [_c128p, _c128p, _c128p, l, i, i]
+        L.oracle_clover_invert.argtypes = [_c128p, _c128p, l, i]
         L.oracle_num_threads.restype = i
         L.oracle_set_num_threads.argtypes = [i]
         _lib = L
@@ -197,6 +209,47 @@ def gauge_eo_reverse(eo, L, Nc):
     lex = np.empty(4 * 2 * half_volume(L) * Nc * Nc, dtype=np.complex128)
     lib().oracle_gauge_eo(_c(eo).reshape(-1), lex, _L(L), Nc, 1)
     return lex
+
+
+def clover(gauge, L, Nc, csw, kappa):
+    """Clover term A = 1 - kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu} as
+    its two chirality blocks per site: ``[parity 2][Vh][chirality 2][2Nc][2Nc]``,
+    block row/column (spin in the pair {0,1} or {2,3}) * Nc + colour."""
+    Vh = half_volume(L)
+    n = 2 * Nc
+    out = np.empty(2 * Vh * 2 * n * n, dtype=np.complex128)
+    lib().oracle_clover(out, _c(gauge).reshape(-1), _L(L), Nc, float(csw), float(kappa))
+    return out.reshape(2, Vh, 2, n, n)
+
+
+def clover_apply(blocks, inp, Nc, R):
+    """A in on one parity: ``blocks`` [Vh][2][2Nc][2Nc], ``inp`` a half
+    vector [Vh][4][Nc][R]."""
+    blocks = _c(blocks)
+    Vh = blocks.size // (2 * (2 * Nc) ** 2)
+    out = np.empty(Vh * 4 * Nc * R, dtype=np.complex128)
+    lib().oracle_clover_apply(out, blocks.reshape(-1), _c(inp).reshape(-1), Vh, Nc, R)
+    return out
+
+
+def clover_invert(blocks):
+    """Inverse of every trailing n x n block (Gauss-Jordan, partial pivoting)."""
+    blocks = _c(blocks)
+    n = blocks.shape[-1]
+    out = np.empty_like(blocks)
+    lib().oracle_clover_invert(out.reshape(-1), blocks.reshape(-1), blocks.size // (n * n), n)
+    return out
+
+
+def clover_mat(inp, gauge, clov, L, Nc, R, kappa, dagger=0):
+    """Full-lattice M = A - kappa D on ``[2][Vh][4][Nc][R]`` (A Hermitian, so
+    the dagger flag only reaches D)."""
+    inp = _c(inp).reshape(-1)
+    out = mat(inp, gauge, L, Nc, R, kappa, dagger)
+    h = out.size // 2
+    for p in (0, 1):
+        out[p * h:(p + 1) * h] += clover_apply(clov[p], inp[p * h:(p + 1) * h], Nc, R) - inp[p * h:(p + 1) * h]
+    return out
 
 
 def num_threads() -> int:

@@ -409,15 +409,22 @@ EXPORT void oracle_set_num_threads(int n) {
 /* The reference declares kDslashClover (include/qcu_helper_macro.h:7-11) but
  * getDslash rejects it (src/qcu_interface.cu:97-105): there is no reference
  * algorithm to restate, so this is the textbook Sheikholeslami-Wohlert term
- * in the kappa normalisation of the Wilson operator M = 1 - kappa D:
+ * (Sheikholeslami & Wohlert, Nucl. Phys. B259 (1985) 572; normalisation of
+ * Luescher, Sint, Sommer & Weisz, Nucl. Phys. B478 (1996) 365: the term
+ * c_sw (i/4) sigma_{mu nu} Fhat_{mu nu} added to D_W, Fhat = (Q - Q^+)/8
+ * anti-Hermitian) in the kappa normalisation M = 2 kappa (D_W + m) of the
+ * Wilson operator M = 1 - kappa D, whose forward hop is (1 - gamma_mu) U_mu(x):
  *
  *     M_clover = A - kappa D,
- *     A(x) = 1 + kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
+ *     A(x) = 1 - kappa c_sw sum_{mu<nu} sigma_{mu nu} F_{mu nu}(x),
  *     sigma_{mu nu} = (i/2) [gamma_mu, gamma_nu]   (the kernels' chiral basis),
- *     F_{mu nu}(x) = ((Q - Q^+) - tr(Q - Q^+)/Nc) / (8 i),
+ *     F_{mu nu}(x) = ((Q - Q^+) - tr(Q - Q^+)/Nc) / (8 i)   (Hermitian, = Fhat / i),
  *
  * Q_{mu nu}(x) the sum of the four plaquette leaves at x in the (mu, nu) plane,
- * each starting at x and oriented mu -> nu.  In the chiral basis sigma is block
+ * each starting at x and oriented mu -> nu.  The sign: D_W = Dslash - D^2/2
+ * with D_mu D_mu = Dslash^2 + (i/2) sigma Fhat leaves -(i/4) sigma Fhat at O(a),
+ * which c_sw = 1 cancels; 2 kappa c_sw (i/4) sum_{mu,nu} sigma Fhat
+ * = -kappa c_sw sum_{mu<nu} sigma F.  In the chiral basis sigma is block
  * diagonal in the spin pairs {0, 1} and {2, 3}, so A(x) is two Hermitian
  * 2Nc x 2Nc blocks, stored as clover[parity][Vh][chirality 2][2Nc][2Nc] with
  * row/column index (spin in the pair) * Nc + colour.                        */
@@ -549,7 +556,7 @@ EXPORT void oracle_clover(cplx *clov, const cplx *gauge, const int L[4], int Nc,
                                 for (int b = 0; b < Nc; ++b) {
                                     cplx v = (s == t && a == b) ? 1.0 : 0.0;
                                     for (int q = 0; q < 6; ++q)
-                                        v += kappa * csw * sig[q][2 * ch + s][2 * ch + t] * F[q * Nc * Nc + a * Nc + b];
+                                        v -= kappa * csw * sig[q][2 * ch + s][2 * ch + t] * F[q * Nc * Nc + a * Nc + b];
                                     blk[(s * Nc + a) * n + t * Nc + b] = v;
                                 }
                 }

@@ -88,6 +88,9 @@ def load() -> ctypes.CDLL:
         "qcuClearError": ([], None),
         "qcuInvertEx": ([i, d, ctypes.POINTER(QcuSolveInfo)], i),
         "qcuSetSolverOptions": ([i, d, i], i),
+        "qcuSetCloverCoefficient": ([d], i),
+        "qcuCloverBuild": ([vp, vp, vp, ip4, i, d, d], i),
+        "qcuCloverApply": ([vp, vp, vp, ip4, i, i, i, i, vp, d], i),
         "qcuDeviceMalloc": ([sz], vp),
         "qcuDeviceFree": ([vp], i),
         "qcuMemcpyHtoD": ([vp, vp, sz], i),

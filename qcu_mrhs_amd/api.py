@@ -231,6 +231,14 @@ def set_solver_options(inner_precision=2, delta=0.0, use_preconditioner=False):
     check(lib().qcuSetSolverOptions(int(inner_precision), float(delta), int(bool(use_preconditioner))))
 
 
+def set_clover_coefficient(csw):
+    """c_sw of the clover operator (getDslash(DSLASH_CLOVER, mass))."""
+    check(lib().qcuSetCloverCoefficient(float(csw)))
+
+
+DSLASH_WILSON, DSLASH_CLOVER = 0, 1
+
+
 def finalizeQcu():
     lib().finalizeQcu()
 
@@ -296,3 +304,18 @@ def dslash_mrhs(out, inp, gauge, lattice, n_color, m_rhs, parity, dagger=0, prec
     check(lib().qcuDslashMrhs(_p(out), _p(inp), _p(gauge), L4, int(n_color), int(m_rhs),
                               int(parity), int(dagger), int(precision),
                               _p(xpay) if xpay is not None else None, float(coef)))
+
+
+def clover_build(clover, clover_inv, gauge, lattice, n_color, csw, kappa):
+    """FP64 clover blocks [2][Vh][2][2Nc][2Nc] (and their inverse, if
+    ``clover_inv`` is not None) from an FP64 even/odd gauge field."""
+    L4 = (ctypes.c_int * 4)(*lattice)
+    check(lib().qcuCloverBuild(_p(clover), _p(clover_inv) if clover_inv is not None else None, _p(gauge), L4,
+                               int(n_color), float(csw), float(kappa)))
+
+
+def clover_apply(out, inp, blocks, lattice, n_color, m_rhs, precision=2, clover_precision=2, xpay=None, coef=0.0):
+    """out = A in + coef * xpay on one parity (``blocks`` = that parity's blocks)."""
+    L4 = (ctypes.c_int * 4)(*lattice)
+    check(lib().qcuCloverApply(_p(out), _p(inp), _p(blocks), L4, int(n_color), int(m_rhs), int(precision),
+                               int(clover_precision), _p(xpay) if xpay is not None else None, float(coef)))

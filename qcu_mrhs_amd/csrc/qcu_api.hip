@@ -14,6 +14,7 @@
 //    src/solver/bicgstab_solve.cu:243) and every call is stream ordered on one
 //    library stream with a single synchronisation at the end.
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "clover.h"
 #include "groups.h"
 #include "layout.h"
 #include "lgnn.h"
@@ -55,6 +57,13 @@ struct Context {
     int user_prec = kDouble, iter_prec = kDouble;
     double mass = 0.0, kappa = 0.125;
     bool have_dslash = false;
+    // Wilson-clover operator (getDslash type kQcuDslashClover)
+    bool clover = false;
+    double csw = 1.0;
+    bool clover_dirty = true;        // gauge, kappa or c_sw changed since the last build
+    void *clov64 = nullptr, *cinv64 = nullptr, *clov32 = nullptr, *cinv32 = nullptr;
+    void *cnorm = nullptr;           // one double: norm of the inverse (singularity check)
+    CloverSet clover_set;
     void *gauge64 = nullptr;
     void *gauge_lo[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *bufA = nullptr, *bufB = nullptr;  // full-lattice MRHS vectors, FP64-sized
@@ -90,7 +99,7 @@ struct Context {
         groups.reset();
         solver.reset();
         precond.reset();
-        for (void *p : {gauge64, bufA, bufB}) if (p) (void)hipFree(p);
+        for (void *p : {gauge64, bufA, bufB, clov64, cinv64, clov32, cinv32, cnorm}) if (p) (void)hipFree(p);
         for (void *p : gauge_lo) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -115,6 +124,67 @@ struct Context {
 };
 
 Context *g_ctx = nullptr;
+
+// The clover term of the current (gauge, kappa, c_sw), built on first use:
+// FP64 blocks and inverse for both parities, FP32 copies for the inner solves.
+bool hip_ok(hipError_t e, const char *what);
+ReduceWork g_ops_red;
+bool ops_red_ready();
+
+bool ensure_clover(Context *c) {
+    if (!c->clover) {
+        c->solver->set_clover(nullptr);
+        return true;
+    }
+    if (c->clover_dirty) {
+        const int n = 2 * c->Nc;
+        const long nblocks = 2L * c->g.Vh * 2;
+        const size_t cnt = size_t(nblocks) * n * n;
+        if (!c->clov64) {
+            if (!hip_ok(hipMalloc(&c->clov64, cnt * sizeof(Cx<double>)), "clover alloc") ||
+                !hip_ok(hipMalloc(&c->cinv64, cnt * sizeof(Cx<double>)), "clover alloc") ||
+                !hip_ok(hipMalloc(&c->clov32, cnt * sizeof(Cx<float>)), "clover alloc") ||
+                !hip_ok(hipMalloc(&c->cinv32, cnt * sizeof(Cx<float>)), "clover alloc") ||
+                !hip_ok(hipMalloc(&c->cnorm, sizeof(double)), "clover alloc"))
+                return false;
+        }
+        bool ok = hip_ok(launch_clover_build(c->clov64, c->gauge64, c->g, c->Nc, c->csw, c->kappa, c->stream),
+                         "clover build");
+        ok = ok && hip_ok(launch_clover_invert(c->cinv64, c->clov64, nblocks, n, c->stream), "clover inverse");
+        ok = ok && hip_ok(launch_convert(c->clov32, kSingle, c->clov64, kDouble, long(cnt), nullptr, 1, c->stream),
+                          "clover copy");
+        ok = ok && hip_ok(launch_convert(c->cinv32, kSingle, c->cinv64, kDouble, long(cnt), nullptr, 1, c->stream),
+                          "clover copy");
+        // a singular block (kappa c_sw |F| of order 1) leaves non-finite entries in the inverse
+        double h = 0.0;
+        ok = ok && ops_red_ready() &&
+             hip_ok(launch_norm(static_cast<double *>(c->cnorm), c->cinv64, long(cnt), 1, kDouble, false, g_ops_red, 0,
+                                c->stream), "clover check") &&
+             hip_ok(hipMemcpyAsync(&h, c->cnorm, sizeof(double), hipMemcpyDeviceToHost, c->stream), "copy") &&
+             hip_ok(hipStreamSynchronize(c->stream), "clover");
+        if (!ok) return false;
+        if (!std::isfinite(h)) {
+            set_error(kQcuErrorInvalidValue, "clover term is singular at kappa %.6g, c_sw %.6g", c->kappa, c->csw);
+            return false;
+        }
+        c->clover_set.a64 = c->clov64;
+        c->clover_set.inv64 = c->cinv64;
+        c->clover_set.a32 = c->clov32;
+        c->clover_set.inv32 = c->cinv32;
+        c->clover_set.parity_stride = long(c->g.Vh) * 2 * n * n;
+        c->clover_dirty = false;
+    }
+    c->solver->set_clover(&c->clover_set);
+    return true;
+}
+
+// entry points whose kernels model the Wilson operator only
+bool wilson_only(Context *c, const char *what) {
+    if (!c->clover) return true;
+    set_error(kQcuErrorUnsupported, "%s supports the Wilson operator only (getDslash type %d)", what,
+              kQcuDslashWilson);
+    return false;
+}
 
 Context *ctx_or_error() {
     if (!g_ctx) set_error(kQcuErrorNotInitialized, "Qcu is not initialized (call initGridSize first)");
@@ -147,7 +217,6 @@ bool hip_ok(hipError_t e, const char *what) {
 }
 
 // stateless ops: one workspace for reductions up to 256 RHS
-ReduceWork g_ops_red;
 bool ops_red_ready() {
     if (g_ops_red.partials) return true;
     return hip_ok(reduce_work_alloc(g_ops_red, 256), "reduction workspace");
@@ -223,6 +292,7 @@ void loadQcuGauge(void *gauge, int floatPrecision) {
             (void)hipFree(p);
             p = nullptr;
         }
+    c->clover_dirty = true;
     if (!hip_ok(launch_convert(c->gauge64, kDouble, gauge, floatPrecision, c->gauge_count(), nullptr, 1,
                                c->stream), "loadQcuGauge"))
         return;
@@ -232,7 +302,12 @@ void loadQcuGauge(void *gauge, int floatPrecision) {
 void getDslash(int dslashType, double mass) {
     Context *c = ctx_or_error();
     if (!c) return;
-    if (dslashType != kQcuDslashWilson) return set_error(kQcuErrorUnsupported, "Unsupported dslash type %d", dslashType);
+    if (dslashType != kQcuDslashWilson && dslashType != kQcuDslashClover)
+        return set_error(kQcuErrorUnsupported, "Unsupported dslash type %d", dslashType);
+    if (dslashType == kQcuDslashClover && !clover_supported(c->Nc))
+        return set_error(kQcuErrorUnsupported, "clover term: n_color %d not supported (2, 3, 4, 8)", c->Nc);
+    c->clover = dslashType == kQcuDslashClover;
+    c->clover_dirty = true;
     c->mass = mass;
     c->kappa = 1.0 / (2.0 * (4.0 + mass));
     c->have_dslash = true;
@@ -268,9 +343,21 @@ void mat_Qcu(int daggerFlag) {
     char *A = static_cast<char *>(c->bufA), *B = static_cast<char *>(c->bufB);
     bool ok = hip_ok(launch_gather(A, P, table(c->q_in, 0, c->user_prec), c->user_prec, 2 * n, c->R,
                                    c->stream), "gather");
+    ok = ok && ensure_clover(c);
     for (int p = 0; p < 2 && ok; ++p) {
+        char *in_p = A + p * n * c->R * eb, *out_p = B + p * n * c->R * eb;
+        if (c->clover) {
+            // out_p = A_pp in_p - kappa D_p in_{1-p}  (A Hermitian: the dagger reaches D only)
+            DslashArgs a{out_p, A + (1 - p) * n * c->R * eb, c->gauge(P), nullptr, 1.0, c->g, c->Nc, c->R, p,
+                         daggerFlag ? 1 : 0, P};
+            ok = a.gauge && hip_ok(launch_dslash(a, c->stream), "dslash");
+            ok = ok && hip_ok(launch_clover_apply(out_p, in_p, c->clover_set.term(P, p), out_p, -c->kappa, c->g.Vh,
+                                                  c->Nc, c->R, P, CloverSet::storage(P), c->stream),
+                              "clover");
+            continue;
+        }
         // out_p = in_p - kappa D_p in_{1-p}   (src/qcu_interface.cu:174-255)
-        DslashArgs a{B + p * n * c->R * eb, A + (1 - p) * n * c->R * eb, c->gauge(P), A + p * n * c->R * eb,
+        DslashArgs a{out_p, A + (1 - p) * n * c->R * eb, c->gauge(P), in_p,
                      -c->kappa, c->g, c->Nc, c->R, p, daggerFlag ? 1 : 0, P};
         ok = a.gauge && hip_ok(launch_dslash(a, c->stream), "dslash");
     }
@@ -297,6 +384,11 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
     if (c->opts.inner_prec != kDouble) {
         gs.glo = c->gauge(c->opts.inner_prec);
         gs.lo_prec = c->opts.inner_prec;
+    }
+    if (!ensure_clover(c) || (c->opts.use_precond && !wilson_only(c, "the learned preconditioner"))) {
+        c->q_in.clear();
+        c->q_out.clear();
+        return g_err;
     }
     Preconditioner *pc = nullptr;
     if (c->opts.use_precond) {
@@ -362,6 +454,18 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
 }
 
 void qcuInvert(int max_iteration, double p_max_prec) { (void)qcuInvertEx(max_iteration, p_max_prec, nullptr); }
+
+int qcuSetCloverCoefficient(double csw) {
+    Context *c = ctx_or_error();
+    if (!c) return kQcuErrorNotInitialized;
+    if (!std::isfinite(csw)) {
+        set_error(kQcuErrorInvalidValue, "c_sw must be finite");
+        return g_err;
+    }
+    c->csw = csw;
+    c->clover_dirty = true;
+    return kQcuSuccess;
+}
 
 int qcuSetSolverOptions(int inner_precision, double delta, int use_preconditioner) {
     Context *c = ctx_or_error();
@@ -455,6 +559,37 @@ int qcuDslashMrhs(void *out, const void *in, const void *gauge, const int lattic
     }
     DslashArgs a{out, in, gauge, xpay, coef, make_geom(lattice), n_color, m_rhs, parity, dagger ? 1 : 0, precision};
     return hip_ok(launch_dslash(a, nullptr), "qcuDslashMrhs") ? 0 : g_err;
+}
+
+int qcuCloverBuild(void *clover, void *clover_inv, const void *gauge, const int lattice[4], int n_color,
+                   double csw, double kappa) {
+    if (!clover || !gauge || !lattice || lattice[0] % 2 || !clover_supported(n_color) || !std::isfinite(csw) ||
+        !std::isfinite(kappa)) {
+        set_error(kQcuErrorInvalidValue, "qcuCloverBuild: invalid arguments");
+        return g_err;
+    }
+    for (int i = 0; i < 4; ++i)
+        if (lattice[i] <= 0) {
+            set_error(kQcuErrorInvalidValue, "qcuCloverBuild: invalid lattice");
+            return g_err;
+        }
+    const Geom g = make_geom(lattice);
+    bool ok = hip_ok(launch_clover_build(clover, gauge, g, n_color, csw, kappa, nullptr), "qcuCloverBuild");
+    if (ok && clover_inv)
+        ok = hip_ok(launch_clover_invert(clover_inv, clover, 2L * g.Vh * 2, 2 * n_color, nullptr), "qcuCloverBuild");
+    return ok && hip_ok(hipStreamSynchronize(nullptr), "qcuCloverBuild") ? 0 : g_err;
+}
+
+int qcuCloverApply(void *out, const void *in, const void *blocks, const int lattice[4], int n_color, int m_rhs,
+                   int precision, int clover_precision, const void *xpay, double coef) {
+    if (!out || !in || !blocks || !lattice || lattice[0] % 2 || !clover_supported(n_color) || m_rhs < 1 ||
+        m_rhs > 512 || !valid_storage(precision) || (clover_precision != kDouble && clover_precision != kSingle)) {
+        set_error(kQcuErrorInvalidValue, "qcuCloverApply: invalid arguments");
+        return g_err;
+    }
+    const Geom g = make_geom(lattice);
+    return hip_ok(launch_clover_apply(out, in, blocks, xpay, coef, g.Vh, n_color, m_rhs, precision,
+                                      clover_precision, nullptr), "qcuCloverApply") ? 0 : g_err;
 }
 
 static int blas_axpy(int kind, void *res, const void *x, const void *y, const void *z, const void *a,
@@ -637,6 +772,7 @@ int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, 
         set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
         return g_err;
     }
+    if (!wilson_only(c, "qcuInvertGroups")) return g_err;
     if (n_groups < 1 || n_groups > kMaxGroups || !groups || max_iteration < 1 || !(tol > 0.0)) {
         set_error(kQcuErrorInvalidValue, "qcuInvertGroups: 1..%d groups, max_iteration >= 1, tol > 0", kMaxGroups);
         return g_err;
@@ -775,6 +911,7 @@ int qcuLgnnCollect(int max_iteration, double tol, int n_snapshots, int n_random,
         set_error(kQcuErrorNotInitialized, "Dslash is not initialized (call getDslash)");
         return g_err;
     }
+    if (!wilson_only(c, "qcuLgnnCollect")) return g_err;
     if (n_snapshots < 0 || n_random < 0 || n_snapshots + n_random < 1) {
         set_error(kQcuErrorInvalidValue, "qcuLgnnCollect: need at least one snapshot or random pair");
         return g_err;

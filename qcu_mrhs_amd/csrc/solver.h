@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "blas.h"
+#include "clover.h"
 #include "dslash.h"
 
 namespace qcu {
@@ -60,11 +61,18 @@ public:
                      int max_iter, double tol, const SolverOptions &opt, Preconditioner *prec,
                      SolveResult &res, hipStream_t s);
 
+    // Wilson-clover operator M = A - kappa D (null: Wilson, A = 1).  The fields
+    // must stay alive while set; the odd Schur system becomes
+    // A_oo - kappa^2 D_oe A_ee^{-1} D_eo, b' = b_o + kappa D_oe A_ee^{-1} b_e and
+    // x_e = A_ee^{-1} (b_e + kappa D_eo x_o).
+    void set_clover(const CloverSet *c) { clover_ = c; }
+
     // FP64 full-lattice true residual max_r ||b - M x|| / ||b||.
     hipError_t true_residual(const void *b_full, const void *x_full, const void *g64, double kappa,
                              std::vector<double> &rel, hipStream_t s);
 
-    // out = in - kappa^2 D_oe D_eo in in precision `prec` (tmp: half vector)
+    // out = in - kappa^2 D_oe D_eo in in precision `prec` (tmp: half vector);
+    // with a clover term out = A_oo in - kappa^2 D_oe A_ee^{-1} D_eo in
     hipError_t apply_schur(void *out, const void *in, void *tmp, const void *gauge, int prec,
                            double kappa, int dagger, hipStream_t s);
 
@@ -78,6 +86,9 @@ public:
 
 private:
     hipError_t ensure(int prec);
+    // out = A_pp in + coef y (clover term or its inverse of parity p, storage `prec`)
+    hipError_t clover_apply(void *out, const void *in, const void *blocks, const void *y, double coef, int prec,
+                            hipStream_t s);
     hipError_t dslash(void *out, const void *in, const void *gauge, int prec, int parity,
                       const void *xpay, double coef, int dagger, hipStream_t s);
     // Reliable-update residual from tmp64_ = D_eo x_o: d = b' - (x_o - kappa^2 D_oe tmp64_)
@@ -93,6 +104,8 @@ private:
          *tmp_ = nullptr, *xlo_ = nullptr, *ph_ = nullptr, *sh_ = nullptr;
     // FP64 vectors
     void *bp_ = nullptr, *tmp64_ = nullptr, *v64_ = nullptr;
+    void *ax64_ = nullptr;     // A_oo x_o of the reliable update (clover only)
+    const CloverSet *clover_ = nullptr;
     C64 *st_ = nullptr;        // scalar slots [kNumSlots][R]
     C64 *ones_ = nullptr;      // [R] of 1
     C64 *kap_ = nullptr;       // [R] of kappa

@@ -25,6 +25,12 @@
 // the rescale is folded into the next p update.  Convergence is only declared
 // on the FP64 true residual.  The second FP64 hop of A x writes no vector: its
 // epilogue forms b' - A x, its per-RHS norm and the scaled inner copy directly.
+//
+// Wilson-clover operator M = A - kappa D (set_clover): the same recurrences on
+// the Schur system A_oo - kappa^2 D_oe A_ee^{-1} D_eo with
+// b' = b_o + kappa D_oe A_ee^{-1} b_e and x_e = A_ee^{-1} (b_e + kappa D_eo x_o);
+// the clover blocks and their inverse are applied by two extra passes per
+// Schur operator (FP64 for FP64 vectors, an FP32 copy for the inner ones).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +104,7 @@ void dfree(void *&p) {
 BiCGStab::BiCGStab(const Geom &g, int Nc, int R) : g_(g), Nc_(Nc), R_(R), n_(long(g.Vh) * 4 * Nc) {}
 
 BiCGStab::~BiCGStab() {
-    for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &tmp64_, &v64_})
+    for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &tmp64_, &v64_, &ax64_})
         dfree(*q);
     void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_, *f = part_;
     dfree(a); dfree(b); dfree(c); dfree(d); dfree(e); dfree(f);
@@ -138,6 +144,10 @@ hipError_t BiCGStab::ensure(int prec) {
         if (err != hipSuccess) return err;
         alloc_prec_ = prec;
     }
+    if (clover_ && !ax64_) {
+        ax64_ = dmalloc(half64, err);
+        if (err != hipSuccess) return err;
+    }
     if (prec >= 0 && prec != kDouble && !part_) {
         part_cap_ = dslash_residual_partials(g_, R_);
         part_ = static_cast<double *>(dmalloc(sizeof(double) * size_t(part_cap_), err));
@@ -163,8 +173,20 @@ hipError_t BiCGStab::dslash(void *out, const void *in, const void *gauge, int pr
     return launch_dslash(a, s);
 }
 
+hipError_t BiCGStab::clover_apply(void *out, const void *in, const void *blocks, const void *y, double coef,
+                                  int prec, hipStream_t s) {
+    return launch_clover_apply(out, in, blocks, y, coef, g_.Vh, Nc_, R_, prec, CloverSet::storage(prec), s);
+}
+
 hipError_t BiCGStab::apply_schur(void *out, const void *in, void *tmp, const void *gauge, int prec,
                                  double kappa, int dagger, hipStream_t s) {
+    if (clover_) {
+        // A is Hermitian, so the dagger only reaches the hops
+        QCU_HIP_TRY(dslash(tmp, in, gauge, prec, 0, nullptr, 1.0, dagger, s));
+        QCU_HIP_TRY(clover_apply(tmp, tmp, clover_->inverse(prec, 0), nullptr, 0.0, prec, s));
+        QCU_HIP_TRY(dslash(out, tmp, gauge, prec, 1, nullptr, 1.0, dagger, s));
+        return clover_apply(out, in, clover_->term(prec, 1), out, -kappa * kappa, prec, s);
+    }
     QCU_HIP_TRY(dslash(tmp, in, gauge, prec, 0, nullptr, 1.0, dagger, s));
     QCU_HIP_TRY(dslash(out, tmp, gauge, prec, 1, in, -kappa * kappa, dagger, s));
     return hipSuccess;
@@ -175,7 +197,7 @@ hipError_t BiCGStab::residual_lo(const void *x_o, const void *g64, double kappa,
     a.out = nullptr;
     a.in = tmp64_;
     a.gauge = g64;
-    a.xpay = x_o;
+    a.xpay = clover_ ? ax64_ : x_o;
     a.coef = -kappa * kappa;
     a.g = g_;
     a.Nc = Nc_;
@@ -185,7 +207,7 @@ hipError_t BiCGStab::residual_lo(const void *x_o, const void *g64, double kappa,
     a.prec = kDouble;
     if (32L * R_ <= 65536L)  // 4 R doubles of LDS per workgroup
         return launch_dslash_residual_lo(a, bp_, r_, prec, dscale_ + R_, part_, part_cap_, dn_, s);
-    QCU_HIP_TRY(dslash(v64_, tmp64_, g64, kDouble, 1, x_o, -kappa * kappa, 0, s));
+    QCU_HIP_TRY(dslash(v64_, tmp64_, g64, kDouble, 1, a.xpay, -kappa * kappa, 0, s));
     return launch_residual_lo(dn_, r_, prec, bp_, v64_, dscale_ + R_, n_, R_, red_, 7, s);
 }
 
@@ -197,8 +219,13 @@ hipError_t BiCGStab::true_residual(const void *b_full, const void *x_full, const
     const auto *x = static_cast<const Cx<double> *>(x_full);
     std::vector<double> rr(R_, 0.0), bb(R_, 0.0);
     for (int p = 0; p < 2; ++p) {
-        // tmp = x_p - kappa D_p x_{1-p} = (M x)_p ; v = b_p - tmp
-        QCU_HIP_TRY(dslash(tmp64_, x + (1 - p) * nR, g64, kDouble, p, x + p * nR, -kappa, 0, s));
+        // tmp = A_pp x_p - kappa D_p x_{1-p} = (M x)_p ; v = b_p - tmp
+        const void *ax = x + p * nR;
+        if (clover_) {
+            QCU_HIP_TRY(clover_apply(v64_, x + p * nR, clover_->term(kDouble, p), nullptr, 0.0, kDouble, s));
+            ax = v64_;
+        }
+        QCU_HIP_TRY(dslash(tmp64_, x + (1 - p) * nR, g64, kDouble, p, ax, -kappa, 0, s));
         QCU_HIP_TRY(launch_xsay_norm(dn_, v64_, b + p * nR, tmp64_, n_, R_, kDouble, false, red_, 4, s));
         QCU_HIP_TRY(launch_norm(dn_ + R_, b + p * nR, n_, R_, kDouble, false, red_, 5, s));
         QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * 2 * R_, hipMemcpyDeviceToHost, s));
@@ -250,8 +277,13 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
     res = SolveResult{};
     res.rel_residual.assign(R_, 1.0);
 
-    // b' = b_o + kappa D_oe b_e and its norm (FP64)
-    QCU_HIP_TRY(dslash(bp_, b_e, gauge.g64, kDouble, 1, b_o, kappa, 0, s));
+    // b' = b_o + kappa D_oe b_e (clover: b_o + kappa D_oe A_ee^{-1} b_e) and its norm (FP64)
+    if (clover_) {
+        QCU_HIP_TRY(clover_apply(tmp64_, b_e, clover_->inverse(kDouble, 0), nullptr, 0.0, kDouble, s));
+        QCU_HIP_TRY(dslash(bp_, tmp64_, gauge.g64, kDouble, 1, b_o, kappa, 0, s));
+    } else {
+        QCU_HIP_TRY(dslash(bp_, b_e, gauge.g64, kDouble, 1, b_o, kappa, 0, s));
+    }
     QCU_HIP_TRY(launch_norm(dn_, bp_, n_, R_, kDouble, true, red_, 3, s));
     QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
     QCU_HIP_TRY(hipStreamSynchronize(s));
@@ -356,7 +388,12 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
                 QCU_HIP_TRY(hipGetLastError());
             }
             // FP64 Schur operator on x_o: first hop into tmp64_, second hop fused with b' - A x_o
+            // (clover: tmp64_ = A_ee^{-1} D_eo x_o and the fused hop's xpay A_oo x_o)
             QCU_HIP_TRY(dslash(tmp64_, x_o, gauge.g64, kDouble, 0, nullptr, 1.0, 0, s));
+            if (clover_) {
+                QCU_HIP_TRY(clover_apply(tmp64_, tmp64_, clover_->inverse(kDouble, 0), nullptr, 0.0, kDouble, s));
+                QCU_HIP_TRY(clover_apply(ax64_, x_o, clover_->term(kDouble, 1), nullptr, 0.0, kDouble, s));
+            }
             QCU_HIP_TRY(residual_lo(x_o, gauge.g64, kappa, P, s));
             QCU_HIP_TRY(hipMemcpyAsync(h_, dn_, sizeof(double) * R_, hipMemcpyDeviceToHost, s));
             QCU_HIP_TRY(hipStreamSynchronize(s));
@@ -428,7 +465,11 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
     // x_e = b_e + kappa D_eo x_o.  A mixed solve that ended on a reliable update has
     // D_eo x_o of the final x_o in tmp64_ already (first dslash of that update's FP64
     // Schur operator): an axpy replaces the dslash.
-    if (!last_rt.empty()) {
+    if (clover_) {
+        // x_e = A_ee^{-1} (b_e + kappa D_eo x_o)
+        QCU_HIP_TRY(dslash(x_e, x_o, gauge.g64, kDouble, 0, b_e, kappa, 0, s));
+        QCU_HIP_TRY(clover_apply(x_e, x_e, clover_->inverse(kDouble, 0), nullptr, 0.0, kDouble, s));
+    } else if (!last_rt.empty()) {
         QCU_HIP_TRY(launch_scalar_fill(kap_, C64{kappa, 0.0}, R_, s));
         AxpyArgs ax{kXpay, x_e, b_e, tmp64_, nullptr, kap_, nullptr, nullptr, n_, R_, kDouble};
         QCU_HIP_TRY(launch_axpy(ax, s));
@@ -438,7 +479,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
     double mx = 0.0;
     if (!last_rt.empty() && !opt.verify_residual) {
         // With x_e = b_e + kappa D_eo x_o the even rows of b - M x vanish and the odd rows equal
-        // b' - A x_o, which the final reliable update has just computed in FP64: the full-system
+        // b' - A x_o (the same with a clover term), which the final reliable update has just computed in FP64: the full-system
         // residual is that vector; only ||b|| is needed.
         QCU_HIP_TRY(launch_norm(dn_, b_e, n_, R_, kDouble, false, red_, 4, s));
         QCU_HIP_TRY(launch_norm(dn_ + R_, b_o, n_, R_, kDouble, false, red_, 5, s));
