@@ -63,6 +63,12 @@ template <> struct Pk<__half> {
 // i^k z one v_pk_mul_f16.
 template <> struct PkPhase<__half> {
     f16x2 sv;
+    // from a sign vector word already in the form make() builds
+    __device__ __forceinline__ static PkPhase raw(uint32_t word) {
+        PkPhase p;
+        p.sv = __builtin_bit_cast(f16x2, word);
+        return p;
+    }
     __device__ __forceinline__ static PkPhase make(uint32_t mask) {
         PkPhase p;
         // (+-1, +-1) in FP16: 0x3c00 per half, the mask's sign bits on top
@@ -86,6 +92,7 @@ template <> struct PkPhase<__half> {
 // (absent for even k) and the per-lane sign bits
 template <> struct PkPhase<__hip_bfloat16> {
     uint32_t mask;
+    __device__ __forceinline__ static PkPhase raw(uint32_t m) { return make(m); }
     __device__ __forceinline__ static PkPhase make(uint32_t m) {
         PkPhase p;
         p.mask = m;
@@ -147,6 +154,37 @@ __device__ __forceinline__ WordOp word_op(int k) {
 }
 // directions in the order x, z, y, t: those with odd phases (x, z) first
 __host__ __device__ constexpr int mu_of(int i) { return ((i & 1) << 1) | (i >> 1); }
+
+// Sign masks of the projection (p0, p1) and reconstruction (r0, r1) phases i^k of
+// every hop position hh = 0..7 (direction mu_of(hh / 2), forward for even hh),
+// without and with the dagger: [dagger][hh][p0, p1, r0, r1].  A lane reads the
+// row of its hop with one 16-byte load instead of deriving the four masks
+// (about 45 vector instructions per wavefront, paid once per site at one site
+// per wavefront).  m: the masks (BF16 word operations); h: the FP16 sign
+// vectors (+-1, +-1) that PkPhase<__half> multiplies by.
+struct PhaseTab {
+    uint32_t m[2][8][4];
+    uint32_t h[2][8][4];
+};
+__host__ __device__ constexpr uint32_t phase_mask(int k) {
+    return ((uint32_t((k & 1) ^ ((k >> 1) & 1))) << 15) | (uint32_t((k >> 1) & 1) << 31);
+}
+__host__ __device__ constexpr PhaseTab make_phase_tab() {
+    PhaseTab t{};
+    for (int dag = 0; dag < 2; ++dag)
+        for (int hh = 0; hh < 8; ++hh) {
+            const int mu = mu_of(hh >> 1);
+            const bool fwd = (hh & 1) == 0;
+            const int sgn = (fwd != (dag != 0)) ? 2 : 0;
+            t.m[dag][hh][0] = phase_mask((kProjK(mu, 0) + sgn) & 3);
+            t.m[dag][hh][1] = phase_mask((kProjK(mu, 1) + sgn) & 3);
+            t.m[dag][hh][2] = phase_mask((kReconK(mu, 0) + sgn) & 3);
+            t.m[dag][hh][3] = phase_mask((kReconK(mu, 1) + sgn) & 3);
+            for (int j = 0; j < 4; ++j) t.h[dag][hh][j] = 0x3c003c00u | (t.m[dag][hh][j] & 0x80008000u);
+        }
+    return t;
+}
+__constant__ PhaseTab kPhaseTab = make_phase_tab();
 
 __device__ __forceinline__ uint32_t apply(const WordOp &o, uint32_t w) {
     return __builtin_amdgcn_perm(w, w, o.sel) ^ o.mask;
@@ -212,13 +250,15 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         const int hh = st * HPS + grp / GPH;
         const int mu = mu_of(hh >> 1);
         const bool fwd = (hh & 1) == 0;
-        const HopInfo hi = hop_info(mu, fwd, DAG);
+        const u32x4 pm = *reinterpret_cast<const u32x4 *>(std::is_same<S, __half>::value
+                                                              ? kPhaseTab.h[DAG ? 1 : 0][hh]
+                                                              : kPhaseTab.m[DAG ? 1 : 0][hh]);
         mu_s[st] = mu;
         fwd_s[st] = fwd;
-        p0_s[st] = PH::make(word_op(hi.kp0).mask);
-        p1_s[st] = PH::make(word_op(hi.kp1).mask);
-        r0_s[st] = PH::make(word_op(hi.kr0).mask);
-        r1_s[st] = PH::make(word_op(hi.kr1).mask);
+        p0_s[st] = PH::raw(pm[0]);
+        p1_s[st] = PH::raw(pm[1]);
+        r0_s[st] = PH::raw(pm[2]);
+        r1_s[st] = PH::raw(pm[3]);
         // B word from the stored link word U: forward V = U[cb][c'], backward
         // V = conj(U[c'][cb]); real column (Vr, -Vi), imaginary column (Vi, Vr)
         bop_s[st].sel = bre ? kPermId : kPermRot;
@@ -332,6 +372,11 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
             u32x4 a[4], bf;
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
+                if (cc >= CPL) {  // zero padding of the k entries (2 Nc < 8)
+                    a[0][cc] = a[1][cc] = a[2][cc] = a[3][cc] = 0u;
+                    bf[cc] = 0u;
+                    continue;
+                }
                 uint32_t h0, h1, g0, g1;
                 if (odd) {
                     h0 = p0_s[st].template fma<true>(w0[st][cc], wb0[st][cc]);
@@ -348,7 +393,7 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
                 a[1][cc] = h1;
                 a[2][cc] = m02 ? g0 : g1;
                 a[3][cc] = m02 ? g1 : g0;
-                bf[cc] = cc < CPL ? apply(bop_s[st], lw[st][cc]) : 0u;
+                bf[cc] = apply(bop_s[st], lw[st][cc]);
             }
             // links as the A operand, spinors as B: D^T = B^T A^T, so the
             // accumulator rows are (colour, re|im) and its column is this lane's RHS
