@@ -82,6 +82,39 @@ def time_dslash(L, Nc, R, d_src_even, d_gauge64, prec, reps=20):
     return ms
 
 
+def time_clover_apply(L, Nc, R, d_gauge64, csw, kappa, reps=20):
+    """Average kernel time (ms) of the clover application of the FP16 inner solve: out = A_oo in +
+    coef * y on FP16 half vectors with the FP32 blocks of one parity, out = y (the last pass of the
+    clover Schur operator), HIP events on the launch stream."""
+    Vh = fields.half_volume(L)
+    nb = 2 * Nc
+    cnt = 2 * Vh * 2 * nb * nb
+    d_c64 = api.DeviceArray(cnt * 16)
+    api.clover_build(d_c64, None, d_gauge64, L, Nc, csw, kappa)
+    d_c32 = api.DeviceArray(cnt * 8)
+    api.check(api.lib().qcuCopyComplexVector(d_c32.ptr, 1, d_c64.ptr, 2, cnt))
+    d_c64.free()
+    n = Vh * 4 * Nc * R
+    d_in = api.DeviceArray(n * 4)
+    d_y = api.DeviceArray(n * 4)
+    d_in.zero()
+    d_y.zero()
+
+    odd_blocks = d_c32.ptr + (cnt // 2) * 8  # the odd parity's blocks
+    for _ in range(3):
+        api.clover_apply(d_y, d_in, odd_blocks, L, Nc, R, 0, 1, xpay=d_y, coef=-kappa * kappa)
+    api.synchronize()
+    api.check(api.lib().qcuTimerStart())
+    for _ in range(reps):
+        api.clover_apply(d_y, d_in, odd_blocks, L, Nc, R, 0, 1, xpay=d_y, coef=-kappa * kappa)
+    ms = api.lib().qcuTimerStopMs() / reps
+    for d in (d_c32, d_in, d_y):
+        d.free()
+    by = Vh * 2 * nb * nb * 8 + 3 * n * 4
+    return {"kernel": "k_clover_apply<3, Half2Store, float> (FP16 spinors, FP32 blocks, xpay)", "ms": ms,
+            "bytes": by, "GBps": by / ms / 1e6, "frac_of_8TBps": by / ms / 1e6 / HBM_PEAK_GBPS}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "dslash_24c3x48_r16_hbm_traffic.md")
 
 
@@ -289,6 +322,9 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-power", type=int, default=3,
                     help="training objective: residual power K of ||(1 - A N)^K v||^2 (0 = pair loss)")
+    ap.add_argument("--clover-csw", type=float, default=1.0,
+                    help="also time the Wilson-clover solve (plain FP64 and FP16 inner) with this c_sw "
+                         "(0 = skip)")
     ap.add_argument("--dslash-only", action="store_true",
                     help="time only the dslash (all storage precisions) and a plain copy kernel "
                          "of one half vector (counter calibration)")
@@ -431,7 +467,30 @@ def main():
         log(f"mixed {m}: {timed[m]['mean_s']:.3f} s/solve over {args.steps}, {runs[-1]['iterations']} iterations, "
             f"{runs[-1]['reliable_updates']} reliable updates, true residual "
             f"{max(r['true_rel_residual'] for r in runs):.2e}, converged={timed[m]['ok']}")
+    clover = None
+    if args.clover_csw > 0:
+        # the Wilson-clover operator A - kappa D on the same gauge field and sources (a widening of the
+        # path: the reference declares the clover type but implements none)
+        api.getDslash(api.DSLASH_CLOVER, 1.0 / (2.0 * kappa) - 4.0)
+        api.set_clover_coefficient(args.clover_csw)
+        clover = {"csw": args.clover_csw}
+        for name, inner in (("plain_fp64", 2), ("half", 0)):
+            t = timed_runs(inner, False)
+            last = t["runs"][-1]
+            clover[name] = {"mean_s": t["mean_s"], "ok": t["ok"], "iterations": last["iterations"],
+                            "reliable_updates": last["reliable_updates"],
+                            "true_rel_residual": max(r["true_rel_residual"] for r in t["runs"])}
+            log(f"clover (c_sw {args.clover_csw}) {name}: {t['mean_s']:.3f} s/solve, {last['iterations']} iterations, "
+                f"true residual {clover[name]['true_rel_residual']:.2e}, converged={t['ok']}")
+        if clover["half"]["mean_s"] > 0:
+            clover["speedup_half_vs_plain"] = clover["plain_fp64"]["mean_s"] / clover["half"]["mean_s"]
+        api.getDslash(0, 1.0 / (2.0 * kappa) - 4.0)
     api.finalizeQcu()
+    if clover is not None:
+        clover["apply_kernel"] = time_clover_apply(L, Nc, R, d_gauge, args.clover_csw, kappa)
+        ck = clover["apply_kernel"]
+        log(f"clover apply (FP16 spinors, FP32 blocks, xpay): {ck['ms'] * 1e3:.1f} us, {ck['GBps']:.0f} GB/s "
+            f"({100 * ck['frac_of_8TBps']:.1f}% of 8 TB/s)")
 
     good = {m: v for m, v in timed.items() if v["ok"]}
     if good:
@@ -498,6 +557,7 @@ def main():
                     if trained is not None else None),
         "learned_objective": (f"residual power {args.train_power}" if args.train_power else "pair loss")
                              if trained is not None else None,
+        "clover": clover,
     }
     print(json.dumps(out))
 

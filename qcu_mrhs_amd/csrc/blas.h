@@ -90,6 +90,11 @@ enum ScalarSlot : int {
 // alpha = rho / <r0, v>
 hipError_t launch_step_alpha(C64 *st, const void *r0, const void *v, long n, int R, int prec,
                              ReduceWork &w, hipStream_t s);
+// The same finishers from per-RHS partial sums computed elsewhere (the dslash
+// epilogue): partials [nblk][R][4] in the layout of the reductions above,
+// (Re, Im) of <r0, v> for alpha; (Re, Im) of <t, s> and <t, t> for omega.
+hipError_t launch_finish_alpha(C64 *st, int R, const double *partials, int nblk, hipStream_t s);
+hipError_t launch_finish_omega(C64 *st, int R, const double *partials, int nblk, hipStream_t s);
 // s = r - alpha v
 hipError_t launch_step_s(void *sv, const void *r, const void *v, const C64 *st, long n, int R,
                          int prec, hipStream_t s);

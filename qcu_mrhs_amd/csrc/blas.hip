@@ -732,6 +732,22 @@ hipError_t launch_step_alpha(C64 *st, const void *r0, const void *v, long n, int
     })
 }
 
+hipError_t launch_finish_alpha(C64 *st, int R, const double *partials, int nblk, hipStream_t s) {
+    if (R < 1 || R > kReduceThreads || nblk < 1) return hipErrorInvalidValue;
+    AlphaOp<Cx<double>> op{nullptr, nullptr, st, R};
+    hipLaunchKernelGGL(k_reduce_finish<AlphaOp<Cx<double>>>, dim3(1), dim3(kReduceThreads), 0, s, op, nblk, R,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish_omega(C64 *st, int R, const double *partials, int nblk, hipStream_t s) {
+    if (R < 1 || R > kReduceThreads || nblk < 1) return hipErrorInvalidValue;
+    OmegaOp<Cx<double>> op{nullptr, nullptr, st, R};
+    hipLaunchKernelGGL(k_reduce_finish<OmegaOp<Cx<double>>>, dim3(1), dim3(kReduceThreads), 0, s, op, nblk, R,
+                       partials);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_s(void *sv, const void *r, const void *v, const C64 *st, long n, int R,
                          int prec, hipStream_t s) {
     const long N = n * R;

@@ -78,6 +78,21 @@ long dslash_residual_partials(const Geom &g, int R);
 hipError_t launch_dslash_residual_lo(const DslashArgs &a, const void *b, void *lo, int lo_prec, const double *scale,
                                      double *partials, long partials_cap, double *nrm, hipStream_t stream);
 
+// out[x][c] = sum of in[i][c] over the rows i of slice x (nrows rows of `cols` doubles cut into `slices`
+// consecutive slices), fixed summation order
+hipError_t launch_rows_sum(double *out, const double *in, int nrows, int cols, int slices, hipStream_t stream);
+
+// Second hop of the unpreconditioned FP16 / BF16 Schur operator, t = s + coef D in with
+// a.xpay = s, with omega's inner products <t, s>, <t, t> in its epilogue (matrix-core
+// kernel, dagger 0): omega = <t, s> / <t, t> lands in st like launch_step_omega.
+// `partials` holds dslash_omega_partials(g, R) doubles.  Returns hipErrorNotSupported
+// (and launches nothing) when the tuned launch for these arguments is not the
+// matrix-core kernel; the caller then runs the separate reduction.
+long dslash_omega_partials(const Geom &g, int R);
+hipError_t launch_dslash_omega(const DslashArgs &a, double *partials, long cap, Cx<double> *st, hipStream_t stream);
+hipError_t launch_dslash_mfma_omega(const DslashArgs &a, const DslashTune &t, double *partials, hipStream_t stream,
+                                    int &nblk);
+
 // One launch of the wave-per-site kernel over every group of a batch (grid.y =
 // group): out/in/xpay are batch half vectors (SegTab layout), a.R is ignored,
 // the group's sites per wavefront come from tab.spw, the traversal tile from

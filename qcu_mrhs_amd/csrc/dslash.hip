@@ -13,11 +13,13 @@
 // those lanes.  The wave-per-site kernels live in dslash_wave.hip (VALU) and
 // dslash_mfma.hip (matrix cores).  Every variant has an optional epilogue
 // out = x + coef * D psi (Schur operator, full-lattice M, source preparation).
+#include <algorithm>
 #include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
 
+#include "blas.h"
 #include "dslash_common.h"
 
 namespace qcu {
@@ -268,6 +270,29 @@ hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
         case kBF16: return dispatch_nc<float, __hip_bfloat16>(a, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+constexpr int kDotSlices = 512;  // first-level slices of the per-workgroup partials
+
+long dslash_omega_partials(const Geom &g, int R) {
+    // one row of R x 4 per workgroup (at most one per 4 sites, or 64 per CU for the persistent grid)
+    const long nblk = std::max<long>(div_up(g.Vh, 4), 64L * num_cus());
+    return (nblk + kDotSlices) * R * 4;
+}
+
+hipError_t launch_dslash_omega(const DslashArgs &a, double *partials, long cap, Cx<double> *st, hipStream_t stream) {
+    if (!dslash_supported(a.Nc, a.R) || a.g.Vh <= 0 || (a.prec != kHalf && a.prec != kBF16) || a.dagger ||
+        !a.xpay || !partials || !st || a.R > kReduceThreads)
+        return hipErrorNotSupported;
+    const DslashTune t = dslash_resolve(a);
+    if (t.variant != kDslashMfma) return hipErrorNotSupported;
+    if (cap < dslash_omega_partials(a.g, a.R)) return hipErrorInvalidValue;
+    int nblk = 0;
+    QCU_HIP_TRY(launch_dslash_mfma_omega(a, t, partials, stream, nblk));
+    // [nblk][R][4] -> [kDotSlices][R][4] (after them) -> the finisher sums the slices in order
+    double *p2 = partials + long(nblk) * a.R * 4;
+    QCU_HIP_TRY(launch_rows_sum(p2, partials, nblk, 4 * a.R, kDotSlices, stream));
+    return launch_finish_omega(st, a.R, p2, kDotSlices, stream);
 }
 
 }  // namespace qcu

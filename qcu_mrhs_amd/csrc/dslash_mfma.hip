@@ -202,12 +202,26 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t v, uns
 // all issued back to back before any of them is waited for.  (Computing the
 // row from the traversal index instead costs about 500 scalar instructions
 // per site, which would make the scalar unit the bottleneck.)
-template <typename S, int NC, bool DAG, bool XP>
+//
+// RED (xpay instance only): the epilogue also accumulates omega's inner
+// products <t, s> and <t, t> over the values it stores (rounded to the storage
+// precision, as a separate reduction would read them), for the unpreconditioned
+// t = s - kappa^2 D_oe D_eo s whose xpay vector is s itself, so the partner
+// words are the xpay words already loaded.  (An inner product with a vector the
+// kernel does not otherwise read -- alpha's <r0, v> -- costs more in this
+// memory-latency-bound kernel than the separate pass it would replace: 360 us
+// against 268 + 105 us at 24^3 x 48, 16 RHS.)  Lanes sum over their entries,
+// the four lane groups of an RHS combine by shuffles and the workgroup's
+// wavefronts through LDS; each workgroup writes one row of per-RHS partials
+// [blockIdx.x][R][4], which two small launches reduce in a fixed order.
+template <typename S, int NC, bool DAG, bool XP, bool RED = false>
 __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ gauge,
                                                     const uint32_t *__restrict__ xpay, float coef, int Vh, int R,
                                                     int parity, const int *__restrict__ walk, int spw,
-                                                    unsigned vec_bytes, unsigned gauge_bytes) {
+                                                    unsigned vec_bytes, unsigned gauge_bytes,
+                                                    double *__restrict__ part = nullptr) {
+    static_assert(!RED || (XP && !DAG), "the fused reduction belongs to the xpay hop of the Schur operator");
     using P = Pk<S>;
     constexpr int KH = (2 * NC <= 8) ? 8 : 16;  // k entries per hop
     constexpr int HPS = 32 / KH;                 // hops per MFMA
@@ -286,10 +300,15 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         q_step = wpx;
         q_end = min((x + 1) * chunk, Vh);
     }
-    if (q_first >= q_end) return;  // uniform
+    if constexpr (!RED)
+        if (q_first >= q_end) return;  // uniform
+    // fused inner products: this lane's partial sums (Re, Im of <s, t>, |t|^2)
+    float racc[3] = {0.f, 0.f, 0.f};
     int cur[9];
+    if (q_first < q_end) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) cur[k] = wtab[long(q_first) * 9 + k];
+        for (int k = 0; k < 9; ++k) cur[k] = wtab[long(q_first) * 9 + k];
+    }
 
     for (int q = q_first; q < q_end; q += q_step) {
         const bool more = q + q_step < q_end;
@@ -423,7 +442,15 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
                                 vr += P::re(xw[i][sp]);
                                 vi += P::im(xw[i][sp]);
                             }
-                            bstore(r_out, P::pack(vr, vi), obase, so);
+                            const uint32_t ow = P::pack(vr, vi);
+                            bstore(r_out, ow, obase, so);
+                            if constexpr (RED) {
+                                const float tr = P::re(ow), ti = P::im(ow);
+                                const float ur = P::re(xw[i][sp]), ui = P::im(xw[i][sp]);
+                                racc[0] = fmaf(ur, tr, fmaf(ui, ti, racc[0]));
+                                racc[1] = fmaf(ur, ti, fmaf(-ui, tr, racc[1]));
+                                racc[2] = fmaf(tr, tr, fmaf(ti, ti, racc[2]));
+                            }
                         }
                     }
                 }
@@ -434,6 +461,29 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
         if (more) {
 #pragma unroll
             for (int k = 0; k < 9; ++k) cur[k] = nxt[k];
+        }
+    }
+    if constexpr (RED) {
+        // the four lane groups of RHS m, then the workgroup's four wavefronts (fixed order)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            racc[k] += __shfl_xor(racc[k], 16);
+            racc[k] += __shfl_xor(racc[k], 32);
+        }
+        __shared__ double red[4][16][3];
+        if (lane < 16)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) red[wv][lane][k] = double(racc[k]);
+        __syncthreads();
+        if (wv == 0 && lane < 16 && r0 + lane < R) {
+            double *row = part + (long(blockIdx.x) * R + r0 + lane) * 4;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double t = ((red[0][lane][k] + red[1][lane][k]) + red[2][lane][k]) + red[3][lane][k];
+                // omega wants <t, s> = conj(<s, t>)
+                row[k] = k == 1 ? -t : t;
+            }
+            row[3] = 0.0;
         }
     }
 }
@@ -455,8 +505,37 @@ hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) 
     auto k = a.dagger ? (xp ? k_dslash_mfma<S, NC, true, true> : k_dslash_mfma<S, NC, true, false>)
                       : (xp ? k_dslash_mfma<S, NC, false, true> : k_dslash_mfma<S, NC, false, false>);
     hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g.Vh, a.R, a.parity,
-                       tab.walk, t.spw, vec_bytes, gauge_bytes);
+                       tab.walk, t.spw, vec_bytes, gauge_bytes, nullptr);
     return hipGetLastError();
+}
+
+template <typename S, int NC>
+hipError_t launch_dot(const DslashArgs &a, const DslashTune &t, double *part, hipStream_t stream, int &nblk_out) {
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(a.g, t.by, t.bz, tab, stream));
+    const int nblk = t.spw > 0 ? div_up(a.g.Vh, 4 * t.spw) : -t.spw * num_cus();
+    nblk_out = nblk;
+    dim3 grid(nblk, div_up(a.R, 16)), block(256);
+    auto out = static_cast<uint32_t *>(a.out);
+    auto in = static_cast<const uint32_t *>(a.in);
+    auto gauge = static_cast<const uint32_t *>(a.gauge);
+    auto xp = static_cast<const uint32_t *>(a.xpay);
+    const unsigned vec_bytes = unsigned(size_t(a.g.Vh) * 4 * a.Nc * a.R * 4);
+    const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
+    hipLaunchKernelGGL((k_dslash_mfma<S, NC, false, true, true>), grid, block, 0, stream, out, in, gauge, xp,
+                       float(a.coef), a.g.Vh, a.R, a.parity, tab.walk, t.spw, vec_bytes, gauge_bytes, part);
+    return hipGetLastError();
+}
+
+template <typename S>
+hipError_t dispatch_dot(const DslashArgs &a, const DslashTune &t, double *part, hipStream_t stream, int &nblk) {
+    switch (a.Nc) {
+        case 2: return launch_dot<S, 2>(a, t, part, stream, nblk);
+        case 3: return launch_dot<S, 3>(a, t, part, stream, nblk);
+        case 4: return launch_dot<S, 4>(a, t, part, stream, nblk);
+        case 8: return launch_dot<S, 8>(a, t, part, stream, nblk);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename S>
@@ -474,6 +553,18 @@ hipError_t dispatch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t str
 
 bool dslash_mfma_supported(int Nc, int prec) {
     return (prec == kHalf || prec == kBF16) && (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8);
+}
+
+hipError_t launch_dslash_mfma_omega(const DslashArgs &a, const DslashTune &t, double *partials, hipStream_t stream,
+                                    int &nblk) {
+    if (!dslash_mfma_supported(a.Nc, a.prec) || a.dagger || !a.xpay) return hipErrorInvalidValue;
+    if (t.by <= 0 || t.bz <= 0 || t.spw == 0 || t.spw < -64 || a.g.L[1] % t.by || a.g.L[2] % t.bz)
+        return hipErrorInvalidValue;
+    if (t.spw < 0 && (num_cus() * 4) % kNumXcd != 0) return hipErrorInvalidValue;
+    if (double(a.g.Vh) * 16.0 * a.Nc * a.R >= 2147483647.0 || double(a.g.Vh) * 32.0 * a.Nc * a.Nc >= 2147483647.0)
+        return hipErrorInvalidValue;
+    return a.prec == kHalf ? dispatch_dot<__half>(a, t, partials, stream, nblk)
+                           : dispatch_dot<__hip_bfloat16>(a, t, partials, stream, nblk);
 }
 
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {

@@ -400,6 +400,14 @@ hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStrea
 
 long dslash_residual_partials(const Geom &g, int R) { return (long(div_up(g.Vh, 4)) + kResidualSlices) * R; }
 
+hipError_t launch_rows_sum(double *out, const double *in, int nrows, int cols, int slices, hipStream_t stream) {
+    if (nrows < 1 || cols < 1 || slices < 1) return hipErrorInvalidValue;
+    const int c = cols < 256 ? cols : 256;
+    hipLaunchKernelGGL(k_rows_sum, dim3(slices, div_up(cols, c)), dim3(256), 0, stream, out, in, nrows, cols,
+                       div_up(nrows, slices), false);
+    return hipGetLastError();
+}
+
 hipError_t launch_dslash_residual_lo(const DslashArgs &a, const void *b, void *lo, int lo_prec, const double *scale,
                                      double *partials, long partials_cap, double *nrm, hipStream_t stream) {
     if (a.prec != kDouble || a.dagger || !a.in || !a.gauge || !a.xpay || !b || !lo || !scale || !partials || !nrm ||

@@ -86,6 +86,11 @@ public:
 
 private:
     hipError_t ensure(int prec);
+    // t = A s whose second hop also forms omega's <t, s>, <t, t> and finishes omega into st_
+    // (FP16 / BF16, when the tuned dslash is the matrix-core kernel); fused = false: only t
+    // was computed and the caller runs the separate reduction
+    hipError_t schur_omega(void *out, const void *in, void *tmp, const void *gauge, int prec, double kappa,
+                           bool &fused, hipStream_t s);
     // out = A_pp in + coef y (clover term or its inverse of parity p, storage `prec`)
     hipError_t clover_apply(void *out, const void *in, const void *blocks, const void *y, double coef, int prec,
                             hipStream_t s);
@@ -114,6 +119,8 @@ private:
     double *h_ = nullptr;      // pinned host staging [8][R]
     double *part_ = nullptr;   // per-workgroup partial sums of the fused residual (mixed solves)
     long part_cap_ = 0;
+    double *dotp_ = nullptr;   // per-workgroup partials of the FP16/BF16 Schur hop's fused omega
+    long dotp_cap_ = 0;
     ReduceWork red_;
 };
 

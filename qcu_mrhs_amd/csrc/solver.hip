@@ -106,8 +106,8 @@ BiCGStab::BiCGStab(const Geom &g, int Nc, int R) : g_(g), Nc_(Nc), R_(R), n_(lon
 BiCGStab::~BiCGStab() {
     for (void **q : {&r0_, &r_, &p_, &v_, &s_, &t_, &tmp_, &xlo_, &ph_, &sh_, &bp_, &tmp64_, &v64_, &ax64_})
         dfree(*q);
-    void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_, *f = part_;
-    dfree(a); dfree(b); dfree(c); dfree(d); dfree(e); dfree(f);
+    void *a = st_, *b = ones_, *c = kap_, *d = dn_, *e = dscale_, *f = part_, *h = dotp_;
+    dfree(a); dfree(b); dfree(c); dfree(d); dfree(e); dfree(f); dfree(h);
     if (h_) (void)hipHostFree(h_);
     reduce_work_free(red_);
 }
@@ -143,6 +143,11 @@ hipError_t BiCGStab::ensure(int prec) {
         if (prec != kDouble) xlo_ = dmalloc(half, err);
         if (err != hipSuccess) return err;
         alloc_prec_ = prec;
+    }
+    if ((prec == kHalf || prec == kBF16) && !dotp_) {
+        dotp_cap_ = dslash_omega_partials(g_, R_);
+        dotp_ = static_cast<double *>(dmalloc(sizeof(double) * size_t(dotp_cap_), err));
+        if (err != hipSuccess) return err;
     }
     if (clover_ && !ax64_) {
         ax64_ = dmalloc(half64, err);
@@ -190,6 +195,32 @@ hipError_t BiCGStab::apply_schur(void *out, const void *in, void *tmp, const voi
     QCU_HIP_TRY(dslash(tmp, in, gauge, prec, 0, nullptr, 1.0, dagger, s));
     QCU_HIP_TRY(dslash(out, tmp, gauge, prec, 1, in, -kappa * kappa, dagger, s));
     return hipSuccess;
+}
+
+hipError_t BiCGStab::schur_omega(void *out, const void *in, void *tmp, const void *gauge, int prec, double kappa,
+                                 bool &fused, hipStream_t s) {
+    fused = false;
+    if (clover_ || !dotp_ || (prec != kHalf && prec != kBF16)) return apply_schur(out, in, tmp, gauge, prec, kappa, 0, s);
+    QCU_HIP_TRY(dslash(tmp, in, gauge, prec, 0, nullptr, 1.0, 0, s));
+    DslashArgs a;
+    a.out = out;
+    a.in = tmp;
+    a.gauge = gauge;
+    a.xpay = in;
+    a.coef = -kappa * kappa;
+    a.g = g_;
+    a.Nc = Nc_;
+    a.R = R_;
+    a.parity = 1;
+    a.dagger = 0;
+    a.prec = prec;
+    const hipError_t e = launch_dslash_omega(a, dotp_, dotp_cap_, st_, s);
+    if (e == hipSuccess) {
+        fused = true;
+        return hipSuccess;
+    }
+    if (e != hipErrorNotSupported) return e;
+    return dslash(out, tmp, gauge, prec, 1, in, -kappa * kappa, 0, s);
 }
 
 hipError_t BiCGStab::residual_lo(const void *x_o, const void *g64, double kappa, int prec, hipStream_t s) {
@@ -329,8 +360,12 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
             QCU_HIP_TRY(prec->apply(sh_, s_, P, s));
             sin = sh_;
         }
-        QCU_HIP_TRY(apply_schur(t_, sin, tmp_, glo, P, kappa, 0, s));
-        QCU_HIP_TRY(launch_step_omega(st_, t_, s_, n_, R_, P, red_, s));
+        // t = A s and omega = <t, s> / <t, t>: without a preconditioner the second hop's xpay vector
+        // is s, and the matrix-core kernel forms both inner products in its epilogue
+        bool fused = false;
+        if (!prec) QCU_HIP_TRY(schur_omega(t_, sin, tmp_, glo, P, kappa, fused, s));
+        else QCU_HIP_TRY(apply_schur(t_, sin, tmp_, glo, P, kappa, 0, s));
+        if (!fused) QCU_HIP_TRY(launch_step_omega(st_, t_, s_, n_, R_, P, red_, s));
         QCU_HIP_TRY(launch_step_xr(x_it, r_, pin, sin, s_, t_, r0_, st_, n_, R_, P, red_, s));
         QCU_HIP_TRY(hipMemcpyAsync(h_, st_ + kRnorm2 * R_, sizeof(C64) * R_, hipMemcpyDeviceToHost, s));
         QCU_HIP_TRY(hipStreamSynchronize(s));
