@@ -90,14 +90,16 @@ int qcuCopyComplexVector(void *dst, int dst_precision, const void *src, int src_
  * stateless qcuDslashMrhs and the solver alike): variant 0 = automatic
  * (tuning table, else MFMA for FP16/BF16 and wave-per-site VALU otherwise),
  * 1 = lane per (site, rhs), 2 = wavefront per site (VALU), 3 = wavefront per
- * site on MFMA (FP16/BF16 only); by, bz = traversal tile in y and z and spw =
+ * site on MFMA (FP16/BF16 only), 4 = MFMA with the input rows staged in LDS
+ * (FP16/BF16, Nc <= 4; spw = t steps per workgroup, 0 = 12; other shapes fall
+ * back to variant 3 or 2); by, bz = traversal tile in y and z and spw =
  * sites per wavefront (0 = choose; for variant 3 a negative spw = a persistent
  * grid of -spw workgroups per CU whose wavefronts stride through their XCD's
  * share of the sites, -64 <= spw).  No reference counterpart. */
 int qcuDslashTune(int variant, int by, int bz, int spw);
 
 /* Autotune the dslash for (lattice, n_color, m_rhs, precision): times the
- * built-in default shape and every kernel variant over sites-per-wavefront
+ * built-in default shape and kernel variants 1-3 over sites-per-wavefront
  * and traversal-tile candidates, and records the fastest for later automatic
  * launches (the default stays unless a candidate is more than 2 % faster).  best[4] receives
  * {variant, by, bz, spw}, *best_us its kernel time; records (may be NULL,

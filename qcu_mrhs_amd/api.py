@@ -259,7 +259,7 @@ def read_gauge_from_file(gauge, path):
 
 
 # --------------------------------------------------------- stateless ops
-DSLASH_VARIANT = {"auto": 0, "lane_rhs": 1, "wave": 2, "mfma": 3}
+DSLASH_VARIANT = {"auto": 0, "lane_rhs": 1, "wave": 2, "mfma": 3, "lds": 4}
 
 
 def dslash_tune(variant="auto", by=0, bz=0, spw=0):

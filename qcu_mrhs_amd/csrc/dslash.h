@@ -14,7 +14,9 @@ enum DslashVariant : int {
     kDslashAuto = 0,      // choose per precision (and from the tuning table)
     kDslashLaneRhs = 1,   // one lane per (site, rhs), all 8 hops per lane
     kDslashWaveSite = 2,  // one wavefront per site: lanes = 16 rhs x 4 dimensions (VALU)
-    kDslashMfma = 3       // one wavefront per site on MFMA (FP16 / BF16 storage)
+    kDslashMfma = 3,      // one wavefront per site on MFMA (FP16 / BF16 storage)
+    kDslashLds = 4        // MFMA with the input rows staged in LDS: one workgroup per 4-site segment of an
+                          // x-row sweeping `spw` consecutive t (FP16 / BF16, Nc <= 4); chosen explicitly
 };
 
 // Launch shape of the wave-per-site kernels.
@@ -59,6 +61,8 @@ void dslash_set_override(const DslashTune &t);
 hipError_t launch_dslash_wave(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 bool dslash_mfma_supported(int Nc, int prec);
+hipError_t launch_dslash_lds(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
+bool dslash_lds_supported(const Geom &g, int Nc, int prec);
 // FP32 storage on the FP32 matrix cores (dslash_mfma32.hip), also for batched groups
 hipError_t launch_dslash_mfma32(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream);

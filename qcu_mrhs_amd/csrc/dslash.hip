@@ -233,6 +233,13 @@ std::vector<std::pair<std::array<int, 7>, DslashTune>> dslash_tuned_entries() {
 DslashTune dslash_fill_defaults(const DslashArgs &a, DslashTune t) {
     if (t.variant == kDslashAuto) t.variant = dslash_mfma_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
     if (t.variant == kDslashMfma && !dslash_matrix_supported(a.Nc, a.prec)) t.variant = kDslashWaveSite;
+    if (t.variant == kDslashLds && !dslash_lds_supported(a.g, a.Nc, a.prec))
+        t.variant = dslash_matrix_supported(a.Nc, a.prec) ? kDslashMfma : kDslashWaveSite;
+    if (t.variant == kDslashLds) {
+        if (t.spw <= 0) t.spw = 12;  // t steps per workgroup
+        t.by = t.bz = 1;             // unused: the workgroups follow the x-row segments
+        return t;
+    }
     if (t.variant != kDslashLaneRhs) {
         // spw < 0 (persistent grid, -spw workgroups per CU) exists for the FP16/BF16 MFMA kernel only
         if (t.spw == 0 || (t.spw < 0 && !(t.variant == kDslashMfma && dslash_mfma_supported(a.Nc, a.prec))))
@@ -261,6 +268,7 @@ hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
     if (a.g.Vh <= 0) return hipSuccess;
     const DslashTune t = dslash_resolve(a);
     if (t.variant == kDslashWaveSite) return launch_dslash_wave(a, t, stream);
+    if (t.variant == kDslashLds) return launch_dslash_lds(a, t, stream);
     if (t.variant == kDslashMfma)
         return a.prec == kSingle ? launch_dslash_mfma32(a, t, stream) : launch_dslash_mfma(a, t, stream);
     switch (a.prec) {

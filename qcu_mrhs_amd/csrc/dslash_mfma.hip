@@ -488,6 +488,363 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged form (variant kDslashLds, Nc <= 4).  One workgroup = a segment of
+// kLdsSites consecutive output sites of one x-row (fixed y, z; one wavefront
+// per site) x a run of Ts consecutive t x one 16-RHS chunk.  It sweeps t and
+// keeps in LDS
+//   * a ring of three input rows (y, z, t-1), (y, z, t), (y, z, t+1), each the
+//     segment plus one halo site: the x hops of layer t read the middle one,
+//     the t hops the outer ones.  The x neighbours of sites xh = a..a+S-1 are
+//     xh - 1 + p and xh + p, p = (y + z + t + parity) & 1 the row's
+//     checkerboard phase, so a row is staged from site a - 1 + p on;
+//   * the four input rows (y-1, z, t), (y+1, z, t), (y, z-1, t), (y, z+1, t)
+//     over the segment, each serving one hop.
+// The rows travel from global memory straight into LDS (buffer loads with the
+// LDS flag: each lane's word lands at M0 + 4 lane), so staging holds no
+// registers and a layer's loads are issued as one batch: a staged row is laid
+// out in spin planes [spin][site][colour][rhs] of whole 64-word chunks, and
+// every row offset that depends only on the workgroup's position is scalar.
+// Per t step every wavefront reads its operand words for layer t from LDS, the
+// workgroup synchronises, the next layer's rows are issued into the slots that
+// just went out of use while the matrix-core work of layer t runs, and a second
+// synchronisation after their arrival closes the step.  The arithmetic is that
+// of k_dslash_mfma (about 5.25 input rows per output row are read against its
+// 8 neighbour spinors per site); small workgroups let several share a CU, so
+// one workgroup's memory wait overlaps another's arithmetic.
+//
+// Measured at 24^3 x 48, SU(3), 16 RHS, FP16 (Ts = 12): 24 % fewer L2 requests
+// per site than k_dslash_mfma, yet about the same time -- 228 us on the
+// autotuner's synthetic fill, 249-262 us on smooth links and Gaussian sources,
+// against 236-250 us for k_dslash_mfma.  The per-layer step chains an LDS read
+// phase, a link round trip and a workgroup barrier on the staged loads, and 5
+// workgroups per CU (LDS and registers) do not hide that chain as well as the
+// free-running wavefronts of k_dslash_mfma hide their L1 misses.  So this form
+// is selected explicitly (qcuDslashTune / qcuDslashSetTuned / a tuning file),
+// not by the autotuner.
+constexpr int kLdsSites = 4;
+
+using lds_void = __attribute__((address_space(3))) void;
+
+// one 64-word chunk: lane l's word of the buffer at voffset + soffset lands at dst[l]
+__device__ __forceinline__ void lds_chunk(__amdgpu_buffer_rsrc_t r, uint32_t *dst, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)dst, 4, int(voff), int(soff), 0, 0);
+}
+
+typedef unsigned u32x8 __attribute__((ext_vector_type(8)));
+
+// centre-row plane: chunks 0..CC-1 of this wavefront's spin plane (voffsets v[k], row offset sb)
+template <int CC>
+__device__ __forceinline__ void stage_center_plane(__amdgpu_buffer_rsrc_t r, uint32_t *dst, u32x8 v, unsigned sb) {
+#pragma unroll
+    for (int k = 0; k < CC; ++k) lds_chunk(r, dst + 64 * k, v[k], sb);
+}
+
+// side rows: 4 rows x 4 planes x NC chunks; wavefront wv takes jobs q = wv + 4 i (i < 4 NC), job q =
+// (row, plane) q / NC, chunk q % NC, whose chunk (and so voffset sv[i % NC]) repeats with period NC in i.
+// rb[row] = byte offset of the row's first site; rows are ROW_W words apart, planes PS.
+template <int NC, int PS, int ROW_W>
+__device__ __forceinline__ void stage_side_rows(__amdgpu_buffer_rsrc_t r, uint32_t *side, u32x4 sv, int wv,
+                                                unsigned rb0, unsigned rb1, unsigned rb2, unsigned rb3,
+                                                unsigned row_bytes) {
+#pragma unroll
+    for (int i = 0; i < 4 * NC; ++i) {
+        const int q = wv + 4 * i;
+        const int rp = q / NC, ch = q - rp * NC;
+        const int row = rp >> 2, sp = rp & 3;
+        const unsigned rb = row == 0 ? rb0 : (row == 1 ? rb1 : (row == 2 ? rb2 : rb3));
+        lds_chunk(r, side + row * ROW_W + sp * PS + 64 * ch, sv[i % NC], rb + unsigned(sp * NC) * row_bytes);
+    }
+}
+
+template <typename S, int NC, bool DAG, bool XP>
+__global__ __launch_bounds__(64 * kLdsSites) void k_dslash_lds(uint32_t *__restrict__ out,
+                                                              const uint32_t *__restrict__ in,
+                                                              const uint32_t *__restrict__ gauge,
+                                                              const uint32_t *__restrict__ xpay, float coef, Geom g,
+                                                              int R, int parity, int Ts, unsigned vec_bytes,
+                                                              unsigned gauge_bytes) {
+    static_assert(NC <= 4, "one k step per four hops");
+    using P = Pk<S>;
+    using PH = PkPhase<S>;
+    constexpr int STEPS = 2, CPL = NC, SEG = kLdsSites;
+    static_assert(SEG == 4, "one wavefront per spin plane of a staged row");
+    constexpr int EW = NC * 16;                // entries (colour, rhs) of one site in one spin plane
+    constexpr int CEN_E = (SEG + 1) * EW;      // entries of a centre-row plane (SIDE_E = SEG EW = 64 NC)
+    constexpr int CC = (CEN_E + 63) / 64;      // 64-word chunks of a centre-row plane
+    constexpr int PS = CC * 64;                // words of one spin plane (centre and side rows alike)
+    constexpr int ROW_W = 4 * PS;              // words of one staged row
+    __shared__ __align__(16) uint32_t side[4][ROW_W];  // (y-, y+, z-, z+) rows of the current layer
+    __shared__ __align__(16) uint32_t cen[3][ROW_W];   // ring of centre rows
+    const int Lxh = g.Lxh, Ly = g.L[1], Lz = g.L[2], Lt = g.L[3];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // site a + wv of the segment
+    const int m = lane & 15, grp = lane >> 4;
+    const int cb = m >> 1;
+    const bool bre = (m & 1) == 0;
+    const int cbc = cb < NC ? cb : NC - 1;
+    const bool gb1 = grp >= 2, fwd = (grp & 1) == 0;  // this lane's hops: forward for even groups
+    const int r0 = 16 * int(blockIdx.y);
+    const int r = r0 + m;
+    const unsigned row_bytes = unsigned(R) * 4u;
+    const unsigned site_bytes = 4u * NC * row_bytes;
+    const unsigned ll4 = 4u * NC * NC;
+    const auto r_in = rsrc(in, vec_bytes);
+    const auto r_g = rsrc(gauge, gauge_bytes);
+    const auto r_out = rsrc(out, vec_bytes);
+    const auto r_x = rsrc(xpay, xpay ? vec_bytes : 0u);
+
+    // workgroup -> (x segment, y, z, t run): consecutive workgroups on one XCD take adjacent segments
+    // and rows, which share their side and halo rows in its L2
+    const int nxs = (Lxh + SEG - 1) / SEG;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int xs = b % nxs;
+    const int y = (b / nxs) % Ly;
+    const int z = (b / (nxs * Ly)) % Lz;
+    const int seg = b / (nxs * Ly * Lz);
+    const int a0 = xs * SEG;
+    const int t0 = seg * Ts, t1 = min(t0 + Ts, Lt);
+    const int yp = y + 1 == Ly ? 0 : y + 1, ym = y == 0 ? Ly - 1 : y - 1;
+    const int zp = z + 1 == Lz ? 0 : z + 1, zm = z == 0 ? Lz - 1 : z - 1;
+    // (the helpers capture by value: a local whose address reached a closure would stay in scratch
+    // memory next to the LDS-load intrinsic)
+    auto row_base = [=](int yy, int zz, int tt) __attribute__((always_inline)) {  // first site index of a row
+        return ((tt * Lz + zz) * Ly + yy) * Lxh;
+    };
+    auto wrap_t = [=](int tl) __attribute__((always_inline)) {  // tl in [-1, 2 Lt)
+        return tl < 0 ? tl + Lt : (tl >= Lt ? tl - Lt : tl);
+    };
+    auto phase = [=](int tt) __attribute__((always_inline)) { return (y + z + tt + parity) & 1; };
+
+    // hop constants of this lane (the same for every site)
+    PH p0_s[STEPS], p1_s[STEPS], r0_s[STEPS], r1_s[STEPS];
+#pragma unroll
+    for (int st = 0; st < STEPS; ++st) {
+        const int hh = st * 4 + grp;
+        const u32x4 pm = *reinterpret_cast<const u32x4 *>(std::is_same<S, __half>::value
+                                                              ? kPhaseTab.h[DAG ? 1 : 0][hh]
+                                                              : kPhaseTab.m[DAG ? 1 : 0][hh]);
+        p0_s[st] = PH::raw(pm[0]);
+        p1_s[st] = PH::raw(pm[1]);
+        r0_s[st] = PH::raw(pm[2]);
+        r1_s[st] = PH::raw(pm[3]);
+    }
+    WordOp bop;  // B word from the stored link word (see k_dslash_mfma)
+    bop.sel = bre ? kPermId : kPermRot;
+    bop.mask = fwd ? (bre ? 0x80000000u : 0u) : (bre ? 0u : 0x00008000u);
+    // link word offsets of this lane: forward V = U[cb][c'], backward V = conj(U[c'][cb])
+    const unsigned lk_off = unsigned(cbc) * (fwd ? NC : 1) * 4u, lk_step = fwd ? 4u : 4u * NC;
+    // partner spin planes: 3, 2 for the x and y hops (lane groups 0, 1), 2, 3 for z and t (groups 2, 3)
+    const int ob0 = (gb1 ? 2 : 3) * PS, ob1 = (gb1 ? 3 : 2) * PS;
+
+    // ---- staging.  Entry e = (site j, colour c, rhs mm) of a plane, e = (j NC + c) 16 + mm; lane l of a
+    // chunk k loads entry 64 k + l.  RHS beyond R and entries beyond a plane read clamped, valid
+    // addresses into words that are never used.
+    const unsigned st_rm = unsigned(min(r0 + m, R - 1)) * 4u;
+    auto entry_off = [=](int e) __attribute__((always_inline)) { return unsigned((e >> 4) % NC) * row_bytes + st_rm; };
+    // centre rows: wavefront wv loads spin plane wv, chunks 0..CC-1; voffset for a row of phase 0 / 1
+    // (vector registers: an index that becomes constant only after unrolling stays out of memory)
+    static_assert(CC <= 8 && NC <= 4, "staging offsets fit their vectors");
+    u32x8 cv0, cv1;
+#pragma unroll
+    for (int k = 0; k < CC; ++k) {
+        const int e = min(64 * k + lane, CEN_E - 1);
+        const int j = e / EW;
+        cv0[k] = unsigned(((a0 - 1 + j) % Lxh + Lxh) % Lxh) * site_bytes + entry_off(e);
+        cv1[k] = unsigned((a0 + j) % Lxh) * site_bytes + entry_off(e);
+    }
+    // side rows: 4 rows x 4 planes x NC chunks; wavefront wv takes jobs q = wv + 4 i (i < 4 NC), job q =
+    // (row, plane) q / NC, chunk q % NC, whose chunk repeats with period NC in i
+    u32x4 sv;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const int e = 64 * ((wv + 4 * i) % NC) + lane;
+        sv[i] = unsigned(min(a0 + e / EW, Lxh - 1)) * site_bytes + entry_off(e);
+    }
+    auto stage_center = [=](int tl, int slot) __attribute__((always_inline)) {
+        const int tt = wrap_t(tl);
+        const unsigned sb = unsigned(row_base(y, z, tt)) * site_bytes + unsigned(wv * NC) * row_bytes;
+        stage_center_plane<CC>(r_in, &cen[slot][wv * PS], phase(tt) ? cv1 : cv0, sb);
+    };
+    auto stage_side = [=](int tl) __attribute__((always_inline)) {
+        const int tt = wrap_t(tl);
+        stage_side_rows<NC, PS, ROW_W>(r_in, &side[0][0], sv, wv, unsigned(row_base(ym, z, tt)) * site_bytes,
+                                       unsigned(row_base(yp, z, tt)) * site_bytes,
+                                       unsigned(row_base(y, zm, tt)) * site_bytes,
+                                       unsigned(row_base(y, zp, tt)) * site_bytes, row_bytes);
+    };
+    // Workgroup barrier ordering LDS accesses only (no wait for outstanding global stores).  The
+    // staged form first waits for this wavefront's loads into LDS (vmcnt(0) in the gfx9 encoding:
+    // vmcnt 0, expcnt 7, lgkmcnt 15), as an instruction the compiler's own wait tracking sees.
+    auto lds_barrier = []() __attribute__((always_inline)) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+    auto staged_barrier = [=]() __attribute__((always_inline)) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        lds_barrier();
+    };
+    auto read_barrier = [=]() __attribute__((always_inline)) { lds_barrier(); };
+
+    // prologue: layers t0 - 1, t0, t0 + 1 into ring slots 0, 1, 2; side rows of layer t0 (one batch)
+    stage_center(t0 - 1, 0);
+    stage_center(t0, 1);
+    stage_center(t0 + 1, 2);
+    stage_side(t0);
+    staged_barrier();
+
+    const int xh = a0 + wv;
+    const bool live = xh < Lxh;  // wavefront-uniform: a segment may run past the row
+    // per-lane parts of the xpay / output offsets (the site part is scalar)
+    const unsigned lane_out = unsigned(min(r, R - 1)) * 4u + unsigned(2 * grp) * row_bytes;
+    for (int t = t0; t < t1; ++t) {
+        const int sl_m = (t - t0) % 3, sl_0 = (t - t0 + 1) % 3, sl_p = (t - t0 + 2) % 3;
+        const int tf = t + 1 == Lt ? 0 : t + 1, tb = t == 0 ? Lt - 1 : t - 1;
+        // ---- operand words of layer t.  Step 0: lane groups 0, 1 the x hops (middle row, window site
+        // wv + 1 forward / wv), 2, 3 the z hops (side rows); step 1: groups 0, 1 the y hops (side rows),
+        // 2, 3 the t hops (outer rows, window site wv + 1 - p(t +- 1))
+        uint32_t w0[STEPS][4], w1[STEPS][4], wb0[STEPS][4], wb1[STEPS][4];
+        {
+            const uint32_t *a_s0 = gb1 ? &side[fwd ? 3 : 2][wv * EW] : &cen[sl_0][(wv + (fwd ? 1 : 0)) * EW];
+            const uint32_t *a_s1 = gb1 ? &cen[fwd ? sl_p : sl_m][(wv + 1 - phase(fwd ? tf : tb)) * EW]
+                                       : &side[fwd ? 1 : 0][wv * EW];
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st) {
+                const uint32_t *src = (st == 0 ? a_s0 : a_s1) + m;
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    if (cc < CPL) {
+                        w0[st][cc] = src[cc * 16];
+                        w1[st][cc] = src[PS + cc * 16];
+                        wb0[st][cc] = src[ob0 + cc * 16];
+                        wb1[st][cc] = src[ob1 + cc * 16];
+                    } else {
+                        w0[st][cc] = w1[st][cc] = wb0[st][cc] = wb1[st][cc] = 0u;
+                    }
+                }
+            }
+        }
+        read_barrier();
+        // ---- this site's links and xpay words, then the next layer's rows into the freed slots
+        const int site = row_base(y, z, t) + xh;
+        const unsigned sbase = unsigned(site) * site_bytes;
+        uint32_t lw[STEPS][4];
+        uint32_t xw[2][4];
+        if (live) {
+            const int x = 2 * xh + phase(t);
+            const int xhb = x == 0 ? Lxh - 1 : (x - 1) >> 1;
+            const unsigned vh = unsigned(g.Vh);
+            const unsigned f0 = unsigned(parity) * vh + unsigned(site), b0 = unsigned(1 - parity) * vh;
+            // forward U_mu(x) at (mu, parity, site); backward U_mu(x - mu) at (mu, 1 - parity, neighbour)
+            const unsigned gx = fwd ? f0 : b0 + unsigned(row_base(y, z, t) + xhb);
+            const unsigned gz = 4u * vh + (fwd ? f0 : b0 + unsigned(row_base(y, zm, t) + xh));
+            const unsigned gy = 2u * vh + (fwd ? f0 : b0 + unsigned(row_base(ym, z, t) + xh));
+            const unsigned gt = 6u * vh + (fwd ? f0 : b0 + unsigned(row_base(y, z, tb) + xh));
+            const unsigned gl[STEPS] = {gb1 ? gz : gx, gb1 ? gt : gy};
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st) {
+                const unsigned gaddr = gl[st] * ll4 + lk_off;
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc)
+                    lw[st][cc] = cc < CPL ? bload(r_g, gaddr + unsigned(cc) * lk_step, 0u) : 0u;
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp)
+                    xw[i][sp] = (XP && 2 * grp + i < NC) ? bload(r_x, lane_out, sbase + unsigned(sp * NC + i) * row_bytes) : 0u;
+        }
+        // issued on the last step of a run too (rows of the wrapped layer, never read): a conditional
+        // issue would make the wait for this step's links also wait for these loads
+        stage_center(t + 2, sl_m);
+        stage_side(t + 1);
+        uint32_t ow[2][4];
+        if (live) {
+            __builtin_amdgcn_s_setprio(1);
+            f32x4 d[4];
+#pragma unroll
+            for (int sp = 0; sp < 4; ++sp) d[sp] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st) {
+                const bool odd = st == 0;  // step 0 holds the x and z hops (odd phases)
+                u32x4 av[4], bf;
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    if (cc >= CPL) {
+                        av[0][cc] = av[1][cc] = av[2][cc] = av[3][cc] = 0u;
+                        bf[cc] = 0u;
+                        continue;
+                    }
+                    uint32_t h0, h1, g0, g1;
+                    if (odd) {
+                        h0 = p0_s[st].template fma<true>(w0[st][cc], wb0[st][cc]);
+                        h1 = p1_s[st].template fma<true>(w1[st][cc], wb1[st][cc]);
+                        g0 = r0_s[st].template mul<true>(h0);
+                        g1 = r1_s[st].template mul<true>(h1);
+                    } else {
+                        h0 = p0_s[st].template fma<false>(w0[st][cc], wb0[st][cc]);
+                        h1 = p1_s[st].template fma<false>(w1[st][cc], wb1[st][cc]);
+                        g0 = r0_s[st].template mul<false>(h0);
+                        g1 = r1_s[st].template mul<false>(h1);
+                    }
+                    av[0][cc] = h0;
+                    av[1][cc] = h1;
+                    av[2][cc] = gb1 ? g0 : g1;  // spin m(0) is 2 for the z and t hops
+                    av[3][cc] = gb1 ? g1 : g0;
+                    bf[cc] = apply(bop, lw[st][cc]);
+                }
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp) d[sp] = P::mma(bf, av[sp], d[sp]);
+            }
+            __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int sp = 0; sp < 4; ++sp) {
+                    float vr = d[sp][2 * i] * coef, vi = d[sp][2 * i + 1] * coef;
+                    if constexpr (XP) {
+                        vr += P::re(xw[i][sp]);
+                        vi += P::im(xw[i][sp]);
+                    }
+                    ow[i][sp] = P::pack(vr, vi);
+                }
+        }
+        // the next layer is in LDS; then this layer's results go out while the next one is read
+        staged_barrier();
+        if (live && r < R) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                if (2 * grp + i < NC) {
+#pragma unroll
+                    for (int sp = 0; sp < 4; ++sp)
+                        bstore(r_out, ow[i][sp], lane_out, sbase + unsigned(sp * NC + i) * row_bytes);
+                }
+        }
+    }
+}
+
+template <typename S, int NC>
+hipError_t launch_lds(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    const int Ts = t.spw > 0 ? t.spw : 12;
+    const int nseg = (a.g.L[3] + Ts - 1) / Ts;
+    const int nxs = (a.g.Lxh + kLdsSites - 1) / kLdsSites;
+    dim3 grid(nxs * a.g.L[1] * a.g.L[2] * nseg, div_up(a.R, 16)), block(64 * kLdsSites);
+    auto out = static_cast<uint32_t *>(a.out);
+    auto in = static_cast<const uint32_t *>(a.in);
+    auto gauge = static_cast<const uint32_t *>(a.gauge);
+    auto xp = static_cast<const uint32_t *>(a.xpay);
+    const unsigned vec_bytes = unsigned(size_t(a.g.Vh) * 4 * a.Nc * a.R * 4);
+    const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
+    auto k = a.dagger ? (xp ? k_dslash_lds<S, NC, true, true> : k_dslash_lds<S, NC, true, false>)
+                      : (xp ? k_dslash_lds<S, NC, false, true> : k_dslash_lds<S, NC, false, false>);
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g, a.R, a.parity, Ts,
+                       vec_bytes, gauge_bytes);
+    return hipGetLastError();
+}
+
 template <typename S, int NC>
 hipError_t launch(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     GeomTables tab;
@@ -550,6 +907,23 @@ hipError_t dispatch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t str
 }
 
 }  // namespace
+
+bool dslash_lds_supported(const Geom &g, int Nc, int prec) {
+    return (prec == kHalf || prec == kBF16) && Nc >= 2 && Nc <= 4 && g.Lxh >= 1 && g.L[1] >= 2 && g.L[2] >= 2 &&
+           g.L[3] >= 2;
+}
+
+hipError_t launch_dslash_lds(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    if (!dslash_lds_supported(a.g, a.Nc, a.prec) || t.spw < 0) return hipErrorInvalidValue;
+    if (double(a.g.Vh) * 16.0 * a.Nc * a.R >= 2147483647.0 || double(a.g.Vh) * 32.0 * a.Nc * a.Nc >= 2147483647.0)
+        return hipErrorInvalidValue;
+    switch (a.Nc) {
+        case 2: return a.prec == kHalf ? launch_lds<__half, 2>(a, t, stream) : launch_lds<__hip_bfloat16, 2>(a, t, stream);
+        case 3: return a.prec == kHalf ? launch_lds<__half, 3>(a, t, stream) : launch_lds<__hip_bfloat16, 3>(a, t, stream);
+        case 4: return a.prec == kHalf ? launch_lds<__half, 4>(a, t, stream) : launch_lds<__hip_bfloat16, 4>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 bool dslash_mfma_supported(int Nc, int prec) {
     return (prec == kHalf || prec == kBF16) && (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8);

@@ -673,7 +673,7 @@ int qcuColorSpinorScatter(void *const *dst_ptrs, int dst_precision, const void *
                   "scatter") ? 0 : g_err;
 }
 int qcuDslashTune(int variant, int by, int bz, int spw) {
-    if (variant < kDslashAuto || variant > kDslashMfma || by < 0 || bz < 0 || spw < -64 ||
+    if (variant < kDslashAuto || variant > kDslashLds || by < 0 || bz < 0 || spw < -64 ||
         (spw < 0 && variant != kDslashMfma)) {
         set_error(kQcuErrorInvalidValue, "qcuDslashTune: invalid arguments");
         return g_err;
@@ -750,7 +750,7 @@ int qcuDslashTuneLoad(const char *path, int *n_loaded) {
 
 int qcuDslashSetTuned(const int lattice[4], int n_color, int m_rhs, int precision, int variant, int by, int bz,
                       int spw) {
-    if (!lattice || variant < kDslashLaneRhs || variant > kDslashMfma || by < 0 || bz < 0 || spw < -64 ||
+    if (!lattice || variant < kDslashLaneRhs || variant > kDslashLds || by < 0 || bz < 0 || spw < -64 ||
         (spw < 0 && variant != kDslashMfma)) {
         set_error(kQcuErrorInvalidValue, "qcuDslashSetTuned: invalid arguments");
         return g_err;

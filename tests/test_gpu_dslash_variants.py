@@ -1,5 +1,6 @@
 """GPU parity of every dslash kernel variant (lane-per-RHS, wave-per-site
-VALU, wave-per-site MFMA) and launch shape against the FP64 oracle."""
+VALU, wave-per-site MFMA, LDS-staged x-row MFMA) and launch shape against the
+FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -96,7 +97,6 @@ def test_mfma_persistent_grid_production_shape(prec, spw):
         assert _rel(d_out.complex(inp.size, prec), ref) < TOL[prec]
         api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
         assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec]
-# to append to tests/test_gpu_dslash_variants.py
 
 
 @pytest.mark.parametrize("L,Nc,R", [((4, 4, 4, 4), 8, 18), ((4, 4, 4, 8), 4, 5), ((6, 4, 4, 4), 3, 17),
@@ -123,3 +123,80 @@ def test_fp32_matrix_core_dslash_matches_oracle(L, Nc, R):
             assert _rel(d_out.complex(inp.size, S), ref) < TOL[S], (spw, parity, dagger)
             api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=S, xpay=d_xp, coef=-0.13)
             assert _rel(d_out.complex(inp.size, S), xp - 0.13 * ref) < TOL[S], (spw, "xpay")
+
+
+@pytest.mark.parametrize("L,Nc,R,ts", [((8, 4, 4, 6), 3, 17, 4), ((4, 6, 4, 8), 2, 3, 5), ((4, 4, 6, 4), 4, 5, 1),
+                                       ((12, 4, 4, 8), 3, 16, 3), ((36, 4, 4, 4), 3, 16, 2),
+                                       ((20, 4, 4, 6), 4, 32, 6), ((6, 4, 4, 4), 3, 12, 48),
+                                       ((2, 4, 2, 4), 3, 4, 3), ((10, 2, 4, 2), 2, 20, 1)])
+def test_lds_staged_dslash_matches_oracle(L, Nc, R, ts):
+    """LDS-staged MFMA kernel (a workgroup per 4-site segment of an x-row,
+    sweeping `ts` t steps with a ring of three haloed full rows and four
+    spin-projected side rows): ragged last t runs, Ts = 1 and Ts beyond Lt,
+    segments running past the row (Lx/2 = 18, 10, 5) and rows shorter than a
+    segment (Lx/2 = 1, 3), Nc 2/3/4, partial 16-RHS chunks, extents of 2,
+    both parities, dagger and the xpay form, FP16 and BF16, against the
+    oracle."""
+    print("case lds", L, Nc, R, ts, flush=True)
+    gauge = fields.gauge_field(L, Nc, 900 + Nc + R, kind="haar")
+    src = fields.gaussian_sources(L, Nc, R, 910 + R).reshape(R, 2, -1)
+    api.dslash_tune("lds", 0, 0, ts)
+    for prec in (H, B):
+        d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+        for parity, dagger in ((0, 0), (1, 1), (1, 0), (0, 1)):
+            inp = fields.to_mrhs(src[:, 1 - parity], Nc)
+            xp = fields.to_mrhs(src[:, parity], Nc)
+            d_in = api.DeviceArray.from_complex(inp, prec)
+            d_xp = api.DeviceArray.from_complex(xp, prec)
+            d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+            d_out.zero()
+            ref = oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec)
+            got = d_out.complex(inp.size, prec)
+            per_site = np.abs(got - ref).reshape(-1, 4 * Nc * R).max(axis=1) / np.abs(ref).max()
+            assert _rel(got, ref) < TOL[prec] and per_site.max() < TOL[prec] * 10, \
+                (prec, parity, dagger, _rel(got, ref), np.argmax(per_site))
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
+            assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec], (prec, parity, dagger, "xpay")
+
+
+def test_lds_staged_dslash_production_shape():
+    """LDS-staged kernel at 24^3 x 48, SU(3), 16 RHS (three segments per
+    x-row) with Ts = 12 and 5, FP16 and BF16, both parities, dagger and xpay,
+    against the oracle."""
+    L, Nc, R = (24, 24, 24, 48), 3, 16
+    gauge = fields.gauge_field(L, Nc, 71, kind="smooth", eps=0.5)
+    src = fields.gaussian_sources(L, Nc, R, 72).reshape(R, 2, -1)
+    cases = []
+    for parity, dagger in ((1, 0), (0, 1)):
+        inp = fields.to_mrhs(src[:, 1 - parity], Nc)
+        xp = fields.to_mrhs(src[:, parity], Nc)
+        cases.append((parity, dagger, inp, xp, oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)))
+    for prec in (H, B):
+        d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+        for ts in (12, 5):
+            print("case lds production", prec, ts, flush=True)
+            api.dslash_tune("lds", 0, 0, ts)
+            for parity, dagger, inp, xp, ref in cases:
+                d_in = api.DeviceArray.from_complex(inp, prec)
+                d_xp = api.DeviceArray.from_complex(xp, prec)
+                d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+                api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec)
+                assert _rel(d_out.complex(inp.size, prec), ref) < TOL[prec], (prec, ts, parity, dagger)
+                api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
+                assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec], (prec, ts, "xpay")
+
+
+def test_lds_variant_falls_back_where_unsupported():
+    """Shapes beyond the LDS kernel's limits (Nc = 8, FP64 and FP32
+    storage) run the wave-per-site kernels instead of failing."""
+    print("case lds fallback", flush=True)
+    api.dslash_tune("lds", 0, 0, 4)
+    for L, Nc, R, prec in (((4, 4, 4, 4), 8, 4, H), ((4, 4, 4, 4), 3, 4, S), ((4, 4, 4, 4), 3, 4, D)):
+        gauge = fields.gauge_field(L, Nc, 5, kind="haar")
+        inp = fields.to_mrhs(fields.gaussian_sources(L, Nc, R, 6).reshape(R, 2, -1)[:, 0], Nc)
+        d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
+        d_in = api.DeviceArray.from_complex(inp, prec)
+        d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
+        api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, 1, 0, precision=prec)
+        assert _rel(d_out.complex(inp.size, prec), oracle.dslash(inp, gauge, L, Nc, R, 1, 0)) < TOL[prec]
