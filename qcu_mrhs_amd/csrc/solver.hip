@@ -17,10 +17,14 @@
 //
 // Mixed method (inner precision FP32/FP16/BF16): the same recurrence on inner
 // vectors stored in that precision (FP32 arithmetic, FP64 reductions) with
-// reliable updates (Sleijpen & van der Vorst 1996): whenever the iterated
+// reliable updates (Sleijpen & van der Vorst 1996): when the iterated
 // residual drops below delta times its maximum since the last update, the
 // inner solution is folded into the FP64 solution and the FP64 true residual
-// b' - A x replaces the iterated one.  The inner vectors are kept at unit
+// b' - A x replaces the iterated one.  An update serves every RHS at once, so
+// where delta leaves room above the inner precision's rounding (FP16, FP32) it
+// waits until every RHS has dropped below delta -- or one has gone 8x below it --
+// instead of updating as soon as the fastest one has (BF16, whose rounding unit
+// is close to delta).  The inner vectors are kept at unit
 // norm per RHS (scale sigma = 1/||r_true||), so FP16 never leaves its range;
 // the rescale is folded into the next p update.  Convergence is only declared
 // on the FP64 true residual.  The second FP64 hop of A x writes no vector: its
@@ -48,17 +52,20 @@ template <> struct SPrec<kSingle> { using C = Cx<float>; };
 template <> struct SPrec<kHalf> { using C = Half2Store; };
 template <> struct SPrec<kBF16> { using C = Bf162Store; };
 
-// x_hi += scale[r] * x_lo
+// x_hi += scale[r] * x_lo.  A block holds a whole number of RHS rows of R threads, so a thread keeps
+// one RHS (and its scale) for the whole grid-stride loop: no per-entry 64-bit remainder.
 template <typename CS>
 __global__ __launch_bounds__(256) void k_accum(Cx<double> *__restrict__ xhi, const CS *__restrict__ xlo,
                                               const double *__restrict__ scale, long N, int R) {
-    const long stride = long(gridDim.x) * blockDim.x;
-    for (long e = long(blockIdx.x) * blockDim.x + threadIdx.x; e < N; e += stride) {
-        const int r = int(e % R);
+    const int per_block = (int(blockDim.x) / R) * R;
+    if (int(threadIdx.x) >= per_block) return;
+    const long stride = long(gridDim.x) * per_block;
+    const double sc = scale[int(threadIdx.x) % R];
+    for (long e = long(blockIdx.x) * per_block + threadIdx.x; e < N; e += stride) {
         const Cx<double> v = load_cx<double>(xlo + e);
         Cx<double> h = xhi[e];
-        h.re = fma(scale[r], v.re, h.re);
-        h.im = fma(scale[r], v.im, h.im);
+        h.re = fma(sc, v.re, h.re);
+        h.im = fma(sc, v.im, h.im);
         xhi[e] = h;
     }
 }
@@ -277,9 +284,12 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
     const int P = opt.inner_prec;
     const bool mixed = P != kDouble;
     // reliable-update threshold: explicit, or a few rounding units of the inner precision
-    // (measured at 24^3x48: fewer updates with no extra iterations down to these values)
-    const double delta = opt.delta > 0.0 ? opt.delta
-                                          : (P == kSingle ? 1e-3 : P == kHalf ? 5e-3 : 1e-2);
+    // (measured at 24^3x48, 16 RHS: FP16 with the all-RHS trigger 1e-2 -> 17 iterations, 4 updates,
+    // 49.2 ms; 5e-3 -> 20 iterations; the first-RHS trigger at 5e-3 -> 17 iterations, 5 updates,
+    // 52.2 ms; BF16 only with the first-RHS trigger: 17 iterations against 20-23)
+    const double delta = opt.delta > 0.0 ? opt.delta : (P == kSingle ? 1e-3 : 1e-2);
+    const double unit = P == kSingle ? 0x1p-24 : P == kHalf ? 0x1p-11 : P == kBF16 ? 0x1p-8 : 0x1p-53;
+    const bool all_rhs = delta >= 16.0 * unit;  // wait for the slowest RHS (see the file header)
     if (mixed && (gauge.glo == nullptr || gauge.lo_prec != P)) {
         set_error(2, "mixed solve needs the gauge field in the inner precision");
         return hipErrorInvalidValue;
@@ -371,7 +381,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
         QCU_HIP_TRY(hipStreamSynchronize(s));
         res.iterations = it + 1;
         if (iterate_hook && !mixed) QCU_HIP_TRY(iterate_hook(it + 1, x_o, s));
-        bool conv_est = true, bad = false, want_update = false;
+        bool conv_est = true, bad = false, all_below = true, one_far_below = false;
         for (int r = 0; r < R_; ++r) {
             const double r2 = h_[2 * r];
             rn[r] = std::sqrt(r2) / sigma[r];
@@ -379,8 +389,10 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
             const double rel = nb[r] > 0 ? rn[r] / nb[r] : 0.0;
             res.rel_residual[r] = rel;
             if (!(rel <= tol)) conv_est = false;
-            if (rn[r] < delta * maxr[r]) want_update = true;
+            if (!(rn[r] < delta * maxr[r])) all_below = false;
+            if (rn[r] < (all_rhs ? 0.125 : 1.0) * delta * maxr[r]) one_far_below = true;
         }
+        const bool want_update = (all_rhs && all_below) || one_far_below;
         if (opt.verbose)
             fprintf(stderr, "[qcu] iter %d  max rel residual %.3e\n", it + 1,
                     *std::max_element(res.rel_residual.begin(), res.rel_residual.end()));
@@ -413,6 +425,7 @@ hipError_t BiCGStab::solve(const void *b_full, void *x_full, const GaugeSet &gau
             }
             QCU_HIP_TRY(hipMemcpyAsync(dscale_, h_, sizeof(double) * 2 * R_, hipMemcpyHostToDevice, s));
             {
+                if (R_ > 256) return hipErrorInvalidValue;  // k_accum holds whole RHS rows per block
                 const int nb_ = div_up(nR, 256) > 8192 ? 8192 : div_up(nR, 256);
                 switch (P) {
                     case kSingle: hipLaunchKernelGGL(k_accum<Cx<float>>, dim3(nb_), dim3(256), 0, s, x_o, static_cast<const Cx<float> *>(xlo_), dscale_, nR, R_); break;
