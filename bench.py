@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--dslash-only", action="store_true",
                     help="time only the dslash (all storage precisions) and a plain copy kernel "
                          "of one half vector (counter calibration)")
-    ap.add_argument("--dslash-variant", default="auto", choices=["auto", "lane_rhs", "wave", "mfma"],
+    ap.add_argument("--dslash-variant", default="auto", choices=["auto", "lane_rhs", "wave", "mfma", "lds"],
                     help="dslash kernel variant for every launch (default: automatic choice)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the dslash launch-shape autotuner (built-in default shapes)")

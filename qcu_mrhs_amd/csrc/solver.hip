@@ -53,19 +53,38 @@ template <> struct SPrec<kHalf> { using C = Half2Store; };
 template <> struct SPrec<kBF16> { using C = Bf162Store; };
 
 // x_hi += scale[r] * x_lo.  A block holds a whole number of RHS rows of R threads, so a thread keeps
-// one RHS (and its scale) for the whole grid-stride loop: no per-entry 64-bit remainder.
+// one RHS (and its scale) for the whole grid-stride loop.  Four entries per pass, all loads issued
+// before the first use: with one entry a wavefront had 1.3 KB in flight and the pass ran at 2.7 TB/s.
 template <typename CS>
 __global__ __launch_bounds__(256) void k_accum(Cx<double> *__restrict__ xhi, const CS *__restrict__ xlo,
                                               const double *__restrict__ scale, long N, int R) {
+    constexpr int U = 4;
     const int per_block = (int(blockDim.x) / R) * R;
     if (int(threadIdx.x) >= per_block) return;
     const long stride = long(gridDim.x) * per_block;
     const double sc = scale[int(threadIdx.x) % R];
-    for (long e = long(blockIdx.x) * per_block + threadIdx.x; e < N; e += stride) {
-        const Cx<double> v = load_cx<double>(xlo + e);
+    long e = long(blockIdx.x) * per_block + threadIdx.x;
+    for (; e + (U - 1) * stride < N; e += U * stride) {
+        CS v[U];
+        Cx<double> h[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            v[k] = xlo[e + k * stride];
+            h[k] = xhi[e + k * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const Cx<double> w = load_cx<double>(&v[k]);
+            h[k].re = fma(sc, w.re, h[k].re);
+            h[k].im = fma(sc, w.im, h[k].im);
+            xhi[e + k * stride] = h[k];
+        }
+    }
+    for (; e < N; e += stride) {
+        const Cx<double> w = load_cx<double>(xlo + e);
         Cx<double> h = xhi[e];
-        h.re = fma(sc, v.re, h.re);
-        h.im = fma(sc, v.im, h.im);
+        h.re = fma(sc, w.re, h.re);
+        h.im = fma(sc, w.im, h.im);
         xhi[e] = h;
     }
 }

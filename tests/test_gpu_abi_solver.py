@@ -106,6 +106,31 @@ def test_invert_small_mixed_precision(inner):
     assert max(err) < 1e-8
 
 
+@pytest.mark.parametrize("inner", [0, 4])
+def test_invert_mixed_with_lds_staged_dslash(inner):
+    """The FP16 / BF16 mixed solve with the LDS-staged dslash selected for
+    every hop (variant 4; the Schur operator then forms omega's inner products
+    in a separate pass): FP64 true residual <= 1e-10 through the oracle
+    operator and the solution within 1e-8 of the oracle's."""
+    L, Nc, R, kappa, tol = (8, 8, 8, 8), 3, 4, 0.12, 1e-10
+    _case("invert mixed lds", inner)
+    gauge = fields.gauge_field(L, Nc, 1001, kind="haar")
+    b = fields.gaussian_sources(L, Nc, R, 1002)
+    api.dslash_tune("lds", 0, 0, 3)
+    try:
+        x, info = _solve(L, Nc, R, gauge, b, kappa, tol, inner=inner)
+    finally:
+        api.dslash_tune("auto")
+    xo, ok, it, _ = oracle.bicgstab(fields.to_mrhs(b, Nc), gauge, L, Nc, R, kappa, 3000, tol)
+    r = fields.to_mrhs(b, Nc) - oracle.mat(fields.to_mrhs(x, Nc), gauge, L, Nc, R, kappa)
+    true_rel = oracle.norm(r, R) / oracle.norm(fields.to_mrhs(b, Nc), R)
+    err = [np.linalg.norm(x[k] - fields.from_mrhs(xo, R)[k]) / np.linalg.norm(x[k]) for k in range(R)]
+    print(f"inner precision {inner} (LDS dslash): {info['iterations']} iterations, {info['reliable_updates']} "
+          f"reliable updates, true residual {true_rel.max():.3e}, per-RHS error vs FP64 oracle {np.max(err):.2e}")
+    assert info["converged"] and true_rel.max() < 1.5e-10
+    assert max(err) < 1e-8
+
+
 @pytest.mark.parametrize("Nc,R,inner", [(2, 17, 0), (4, 17, 4), (3, 33, 0)])
 def test_invert_mixed_precision_rhs_chunks(Nc, R, inner):
     """Matrix-core inner operator over more than one 16-RHS chunk (R = 17, 33:
