@@ -287,12 +287,15 @@ def bench_groups(args):
         wall = float(np.mean([r[0] for r in runs]))
         info = runs[-1][1]
         variant = {2: "VALU wave per site", 3: "FP32 matrix cores"}.get(api.lib().qcuGroupsDslashVariant(), "?")
+        variant64 = {2: "VALU wave per site", 3: "FP64 matrix cores"}.get(api.lib().qcuGroupsDslashVariant64(), "?")
         res = {"batch_s": wall, "rhs_per_s": sum(sizes) / wall, "dslash_kernel": variant,
+               "dslash_kernel_fp64": variant64,
                "groups": [{"m_rhs": R, "converged": bool(i["converged"]), "iterations": i["iterations"],
                            "outer_steps": i["outer_steps"], "sites_per_wave": i["sites_per_wave"],
                            "solve_s": i["seconds"], "true_rel_residual": i["true_rel_residual"]}
                           for R, i in zip(sizes, info)]}
-        log(f"SU({Nc}): batch {wall * 1e3:.1f} ms, {res['rhs_per_s']:.1f} rhs/s, {variant}; " + ", ".join(
+        log(f"SU({Nc}): batch {wall * 1e3:.1f} ms, {res['rhs_per_s']:.1f} rhs/s, {variant} (FP64 hops: {variant64}); "
+            + ", ".join(
             f"R={g['m_rhs']}: {g['iterations']} it, {g['solve_s'] * 1e3:.1f} ms, {g['true_rel_residual']:.1e}"
             for g in res["groups"]))
         out["results"][f"su{Nc}"] = res

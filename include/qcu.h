@@ -216,6 +216,10 @@ int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, 
  * (v_mfma_f32_16x16x4_f32, FP32 storage); 0 before any batch, -1 without
  * initGridSize. */
 int qcuGroupsDslashVariant(void);
+/* The same for the FP64 hops of the batch's defect correction: 2 = VALU wave
+ * per site, 3 = FP64 matrix cores (v_mfma_f64_16x16x4_f64); timed once per
+ * batch shape.  No reference counterpart. */
+int qcuGroupsDslashVariant64(void);
 
 /* Device-memory plumbing for callers that have no other GPU runtime. */
 void *qcuDeviceMalloc(size_t bytes);

@@ -116,6 +116,7 @@ def load() -> ctypes.CDLL:
         "qcuDslashTuneLoad": ([ctypes.c_char_p, ip4], i),
         "qcuInvertGroups": ([i, ctypes.POINTER(QcuRhsGroup), i, d, ctypes.POINTER(QcuGroupInfo)], i),
         "qcuGroupsDslashVariant": ([], i),
+        "qcuGroupsDslashVariant64": ([], i),
         "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnApply": ([vp, vp, i], i),

@@ -63,13 +63,18 @@ hipError_t launch_dslash_mfma(const DslashArgs &a, const DslashTune &t, hipStrea
 bool dslash_mfma_supported(int Nc, int prec);
 hipError_t launch_dslash_lds(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 bool dslash_lds_supported(const Geom &g, int Nc, int prec);
-// FP32 storage on the FP32 matrix cores (dslash_mfma32.hip), also for batched groups
+// FP32 / FP64 storage on the FP32 / FP64 matrix cores (dslash_mfma32.hip), also for batched groups
 hipError_t launch_dslash_mfma32(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream);
 bool dslash_mfma32_supported(int Nc, int prec);
-// either matrix-core kernel exists for (Nc, prec)
+bool dslash_mfma64_supported(int Nc, int prec);
+// a matrix-core kernel exists for (Nc, prec)
 inline bool dslash_matrix_supported(int Nc, int prec) {
-    return dslash_mfma_supported(Nc, prec) || dslash_mfma32_supported(Nc, prec);
+    return dslash_mfma_supported(Nc, prec) || dslash_mfma32_supported(Nc, prec) || dslash_mfma64_supported(Nc, prec);
+}
+// the group kernels exist on matrix cores for (Nc, prec): FP32 or FP64 storage
+inline bool dslash_groups_matrix_supported(int Nc, int prec) {
+    return dslash_mfma32_supported(Nc, prec) || dslash_mfma64_supported(Nc, prec);
 }
 
 // Second hop of the FP64 Schur operator fused with the reliable-update residual

@@ -270,7 +270,8 @@ hipError_t launch_dslash(const DslashArgs &a, hipStream_t stream) {
     if (t.variant == kDslashWaveSite) return launch_dslash_wave(a, t, stream);
     if (t.variant == kDslashLds) return launch_dslash_lds(a, t, stream);
     if (t.variant == kDslashMfma)
-        return a.prec == kSingle ? launch_dslash_mfma32(a, t, stream) : launch_dslash_mfma(a, t, stream);
+        return (a.prec == kSingle || a.prec == kDouble) ? launch_dslash_mfma32(a, t, stream)
+                                                         : launch_dslash_mfma(a, t, stream);
     switch (a.prec) {
         case kDouble: return dispatch_nc<double, double>(a, stream);
         case kSingle: return dispatch_nc<float, float>(a, stream);

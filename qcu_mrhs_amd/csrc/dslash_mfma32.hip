@@ -1,5 +1,6 @@
-// dslash_mfma32.hip -- Wilson hopping term on the FP32 matrix cores
-// (v_mfma_f32_16x16x4_f32) for FP32 storage, Nc = 2, 3, 4, 8.
+// dslash_mfma32.hip -- Wilson hopping term on the FP32 and FP64 matrix cores
+// (v_mfma_f32_16x16x4_f32 for FP32 storage, v_mfma_f64_16x16x4_f64 for FP64
+// storage: the same 16 x 16 x 4 tile and operand layout), Nc = 2, 3, 4, 8.
 //
 // One wavefront computes one output site for a chunk of 16 right-hand sides
 // (lane l: RHS m = l % 16, lane group g = l / 16).  Per hop and projected row
@@ -12,8 +13,9 @@
 // (re, im) rows), a second the imaginary parts (A = (-Im V, Re V)), so each
 // lane loads every complex number it needs once (input colour c' = 4 cg + g,
 // output row m).  Operand layout of the instruction: lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[4 (l / 16) + v][l % 16],
-// v = 0..3, i.e. output colours 2g, 2g + 1 (re, im) of RHS m -- whole complex
+// A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[4 (l / 16) + v][l % 16]
+// (FP32; the FP64 instruction returns D[l / 16 + 4 v][l % 16], and its A rows
+// are permuted to match), v = 0..3, i.e. output colours 2g, 2g + 1 (re, im) of RHS m -- whole complex
 // numbers per lane, so the reconstruction of the lower spins (a phase i^k per
 // hop, known at compile time) is register arithmetic: every hop's product
 // starts from zero and is added to the upper spin and, with its phase, to the
@@ -22,6 +24,7 @@
 // Entries beyond Nc (rows or input colours) and RHS beyond R read clamped,
 // valid addresses and are zeroed or never stored.
 #include <algorithm>
+#include <type_traits>
 
 #include "dslash_common.h"
 #include "geom.h"
@@ -30,14 +33,21 @@ namespace qcu {
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T> struct Acc;
+template <> struct Acc<float> { using type = f32x4; };
+template <> struct Acc<double> { using type = f64x4; };
 
 __device__ __forceinline__ f32x4 mma4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f64x4 mma4(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
 // z * i^k for compile-time k
-template <int K>
-__device__ __forceinline__ Cx<float> rot(Cx<float> z) {
+template <int K, typename T>
+__device__ __forceinline__ Cx<T> rot(Cx<T> z) {
     if constexpr ((K & 3) == 0) return z;
     else if constexpr ((K & 3) == 1) return {-z.im, z.re};
     else if constexpr ((K & 3) == 2) return {-z.re, -z.im};
@@ -59,33 +69,34 @@ struct Mfma32Lane {
 };
 
 // One hop (mu, FWD) of one site: accumulates the four spins of acc (lane layout of D).
-template <int NC, bool DAG, int MU, bool FWD>
-__device__ __forceinline__ void mfma32_hop(f32x4 (&acc)[4], const Cx<float> *__restrict__ nbsp,
-                                           const Cx<float> *__restrict__ link, const Mfma32Lane<NC> &ln, long R,
+template <typename T, int NC, bool DAG, int MU, bool FWD>
+__device__ __forceinline__ void mfma32_hop(typename Acc<T>::type (&acc)[4], const Cx<T> *__restrict__ nbsp,
+                                           const Cx<T> *__restrict__ link, const Mfma32Lane<NC> &ln, long R,
                                            long r) {
+    using V = typename Acc<T>::type;
     constexpr int NCG = (NC + 3) / 4;
     constexpr int M0 = kMat2(MU, 0), M1 = kMat2(MU, 1);
     constexpr int KP0 = proj_k(MU, 0, FWD, DAG), KP1 = proj_k(MU, 1, FWD, DAG);
     constexpr int KR0 = recon_k(MU, 0, FWD, DAG), KR1 = recon_k(MU, 1, FWD, DAG);
-    f32x4 p0 = {0.f, 0.f, 0.f, 0.f}, p1 = {0.f, 0.f, 0.f, 0.f};
+    V p0 = {T(0), T(0), T(0), T(0)}, p1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int cg = 0; cg < NCG; ++cg) {
         const int cp = 4 * cg + ln.g;
         const bool kin = cp < NC;
         const int cpc = kin ? cp : NC - 1;
         // projected rows h_0 = psi_0 + i^KP0 psi_M0, h_1 = psi_1 + i^KP1 psi_M1 for (c', rhs)
-        const Cx<float> a0 = nbsp[(0 * NC + cpc) * R + r];
-        const Cx<float> a1 = nbsp[(1 * NC + cpc) * R + r];
-        const Cx<float> b0 = nbsp[(M0 * NC + cpc) * R + r];
-        const Cx<float> b1 = nbsp[(M1 * NC + cpc) * R + r];
-        Cx<float> h0 = a0 + rot<KP0>(b0), h1 = a1 + rot<KP1>(b1);
+        const Cx<T> a0 = nbsp[(0 * NC + cpc) * R + r];
+        const Cx<T> a1 = nbsp[(1 * NC + cpc) * R + r];
+        const Cx<T> b0 = nbsp[(M0 * NC + cpc) * R + r];
+        const Cx<T> b1 = nbsp[(M1 * NC + cpc) * R + r];
+        Cx<T> h0 = a0 + rot<KP0>(b0), h1 = a1 + rot<KP1>(b1);
         // link V[c][c']: forward U[c][c'], backward conj(U[c'][c])
-        Cx<float> v = FWD ? link[ln.c * NC + cpc] : link[cpc * NC + ln.c];
+        Cx<T> v = FWD ? link[ln.c * NC + cpc] : link[cpc * NC + ln.c];
         if (!FWD) v.im = -v.im;
-        float are = ln.part ? v.im : v.re;   // multiplies Re h
-        float aim = ln.part ? v.re : -v.im;  // multiplies Im h
-        if (!(kin && ln.arow)) are = aim = 0.f;
-        if (!kin) h0 = h1 = {0.f, 0.f};
+        T are = ln.part ? v.im : v.re;   // multiplies Re h
+        T aim = ln.part ? v.re : -v.im;  // multiplies Im h
+        if (!(kin && ln.arow)) are = aim = T(0);
+        if (!kin) h0 = h1 = {T(0), T(0)};
         p0 = mma4(are, h0.re, p0);
         p0 = mma4(aim, h0.im, p0);
         p1 = mma4(are, h1.re, p1);
@@ -96,8 +107,8 @@ __device__ __forceinline__ void mfma32_hop(f32x4 (&acc)[4], const Cx<float> *__r
     acc[1] += p1;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-        const Cx<float> q0 = rot<KR0>(Cx<float>{p0[2 * e], p0[2 * e + 1]});
-        const Cx<float> q1 = rot<KR1>(Cx<float>{p1[2 * e], p1[2 * e + 1]});
+        const Cx<T> q0 = rot<KR0>(Cx<T>{p0[2 * e], p0[2 * e + 1]});
+        const Cx<T> q1 = rot<KR1>(Cx<T>{p1[2 * e], p1[2 * e + 1]});
         acc[M0][2 * e] += q0.re;
         acc[M0][2 * e + 1] += q0.im;
         acc[M1][2 * e] += q1.re;
@@ -107,18 +118,27 @@ __device__ __forceinline__ void mfma32_hop(f32x4 (&acc)[4], const Cx<float> *__r
 
 // Sites q of the traversal handled by this wavefront: spw consecutive positions starting at
 // (b * 4 + wavefront) * spw; every chunk of 16 RHS of each site.
-template <int NC, bool DAG>
-__device__ __forceinline__ void mfma32_block(int b, Cx<float> *__restrict__ out, const Cx<float> *__restrict__ in,
-                                             const Cx<float> *__restrict__ gauge, const Cx<float> *__restrict__ xpay,
-                                             float coef, int Vh, int R, int parity, const int *__restrict__ walk,
+template <typename T, int NC, bool DAG>
+__device__ __forceinline__ void mfma32_block(int b, Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
+                                             const Cx<T> *__restrict__ gauge, const Cx<T> *__restrict__ xpay,
+                                             T coef, int Vh, int R, int parity, const int *__restrict__ walk,
                                              int spw) {
+    using V = typename Acc<T>::type;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     Mfma32Lane<NC> ln;
     ln.m = lane & 15;
     ln.g = lane >> 4;
-    ln.c = ln.m >> 1;
-    ln.part = ln.m & 1;
+    if constexpr (std::is_same<T, double>::value) {
+        // FP64 instruction: lane l receives D[l / 16 + 4 v][l % 16] (measured on gfx950), so A row m
+        // holds (colour 2 (m & 3) + (m >> 3), part (m >> 2) & 1) to land colours 2g, 2g + 1 (re, im) in
+        // v = 0..3 of lane group g, as the FP32 layout does with A row m = (colour m >> 1, part m & 1)
+        ln.c = 2 * (ln.m & 3) + (ln.m >> 3);
+        ln.part = (ln.m >> 2) & 1;
+    } else {
+        ln.c = ln.m >> 1;
+        ln.part = ln.m & 1;
+    }
     ln.arow = ln.c < NC;
     if (!ln.arow) ln.c = NC - 1;  // clamped address, A operand zeroed
     const long site_len = 4L * NC * R;
@@ -134,14 +154,14 @@ __device__ __forceinline__ void mfma32_block(int b, Cx<float> *__restrict__ out,
         const int site = nbr[0];
         for (int r0 = 0; r0 < R; r0 += 16) {
             const long r = min(r0 + ln.m, R - 1);
-            f32x4 acc[4];
+            V acc[4];
 #pragma unroll
-            for (int s = 0; s < 4; ++s) acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int s = 0; s < 4; ++s) acc[s] = V{T(0), T(0), T(0), T(0)};
 #define QCU_HOP(MU)                                                                                              \
-    mfma32_hop<NC, DAG, MU, true>(acc, in + nbr[1 + 2 * MU] * site_len,                                          \
-                                  gauge + ((MU * 2 + parity) * long(Vh) + site) * link_len, ln, R, r);           \
-    mfma32_hop<NC, DAG, MU, false>(acc, in + nbr[2 + 2 * MU] * site_len,                                         \
-                                   gauge + ((MU * 2 + 1 - parity) * long(Vh) + nbr[2 + 2 * MU]) * link_len, ln, R, r);
+    mfma32_hop<T, NC, DAG, MU, true>(acc, in + nbr[1 + 2 * MU] * site_len,                                       \
+                                     gauge + ((MU * 2 + parity) * long(Vh) + site) * link_len, ln, R, r);        \
+    mfma32_hop<T, NC, DAG, MU, false>(acc, in + nbr[2 + 2 * MU] * site_len,                                      \
+                                      gauge + ((MU * 2 + 1 - parity) * long(Vh) + nbr[2 + 2 * MU]) * link_len, ln, R, r);
             QCU_HOP(0)
             QCU_HOP(1)
             QCU_HOP(2)
@@ -156,7 +176,7 @@ __device__ __forceinline__ void mfma32_block(int b, Cx<float> *__restrict__ out,
 #pragma unroll
                         for (int s = 0; s < 4; ++s) {
                             const long o = site * site_len + (long(s) * NC + c) * R + r;
-                            Cx<float> v = {coef * acc[s][2 * e], coef * acc[s][2 * e + 1]};
+                            Cx<T> v = {coef * acc[s][2 * e], coef * acc[s][2 * e + 1]};
                             if (xpay) v = v + xpay[o];
                             out[o] = v;
                         }
@@ -167,20 +187,19 @@ __device__ __forceinline__ void mfma32_block(int b, Cx<float> *__restrict__ out,
     }
 }
 
-template <int NC, bool DAG>
-__global__ __launch_bounds__(256) void k_dslash_mfma32(Cx<float> *__restrict__ out, const Cx<float> *__restrict__ in,
-                                                      const Cx<float> *__restrict__ gauge,
-                                                      const Cx<float> *__restrict__ xpay, float coef, int Vh, int R,
-                                                      int parity, const int *__restrict__ walk, int spw) {
-    mfma32_block<NC, DAG>(xcd_remap(blockIdx.x, gridDim.x), out, in, gauge, xpay, coef, Vh, R, parity, walk, spw);
+template <typename T, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_mfma32(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
+                                                      const Cx<T> *__restrict__ gauge, const Cx<T> *__restrict__ xpay,
+                                                      T coef, int Vh, int R, int parity, const int *__restrict__ walk,
+                                                      int spw) {
+    mfma32_block<T, NC, DAG>(xcd_remap(blockIdx.x, gridDim.x), out, in, gauge, xpay, coef, Vh, R, parity, walk, spw);
 }
 
 // every group of a batch in one launch: blockIdx.y = group
-template <int NC, bool DAG>
-__global__ __launch_bounds__(256) void k_dslash_mfma32_groups(Cx<float> *__restrict__ out,
-                                                             const Cx<float> *__restrict__ in,
-                                                             const Cx<float> *__restrict__ gauge,
-                                                             const Cx<float> *__restrict__ xpay, float coef, int Vh,
+template <typename T, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_mfma32_groups(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
+                                                             const Cx<T> *__restrict__ gauge,
+                                                             const Cx<T> *__restrict__ xpay, T coef, int Vh,
                                                              int parity, const int *__restrict__ walk, SegTab tab) {
     const int grp = blockIdx.y;
     if (tab.done && tab.done[grp]) return;
@@ -188,39 +207,61 @@ __global__ __launch_bounds__(256) void k_dslash_mfma32_groups(Cx<float> *__restr
     const int nblk = (Vh + 4 * spw - 1) / (4 * spw);
     if (int(blockIdx.x) >= nblk) return;
     const long off = tab.n * tab.base[grp];
-    mfma32_block<NC, DAG>(xcd_remap(blockIdx.x, nblk), out + off, in + off, gauge, xpay ? xpay + off : nullptr, coef,
-                          Vh, tab.R[grp], parity, walk, spw);
+    mfma32_block<T, NC, DAG>(xcd_remap(blockIdx.x, nblk), out + off, in + off, gauge, xpay ? xpay + off : nullptr,
+                             coef, Vh, tab.R[grp], parity, walk, spw);
 }
 
-template <int NC>
+template <typename T, int NC>
 hipError_t launch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(a.g, t.by, t.bz, tab, stream));
     dim3 grid(div_up(a.g.Vh, 4 * t.spw)), block(256);
-    auto out = static_cast<Cx<float> *>(a.out);
-    auto in = static_cast<const Cx<float> *>(a.in);
-    auto gauge = static_cast<const Cx<float> *>(a.gauge);
-    auto xp = static_cast<const Cx<float> *>(a.xpay);
-    auto k = a.dagger ? k_dslash_mfma32<NC, true> : k_dslash_mfma32<NC, false>;
-    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g.Vh, a.R, a.parity,
-                       tab.walk, t.spw);
+    auto out = static_cast<Cx<T> *>(a.out);
+    auto in = static_cast<const Cx<T> *>(a.in);
+    auto gauge = static_cast<const Cx<T> *>(a.gauge);
+    auto xp = static_cast<const Cx<T> *>(a.xpay);
+    auto k = a.dagger ? k_dslash_mfma32<T, NC, true> : k_dslash_mfma32<T, NC, false>;
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, T(a.coef), a.g.Vh, a.R, a.parity, tab.walk,
+                       t.spw);
     return hipGetLastError();
 }
 
-template <int NC>
+template <typename T, int NC>
 hipError_t launch_groups_nc(const DslashArgs &a, const SegTab &seg, int by, int bz, hipStream_t stream) {
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(a.g, by, bz, tab, stream));
     int nmax = 1;
     for (int k = 0; k < seg.G; ++k) nmax = std::max(nmax, div_up(a.g.Vh, 4 * seg.spw[k]));
     dim3 grid(nmax, seg.G), block(256);
-    auto out = static_cast<Cx<float> *>(a.out);
-    auto in = static_cast<const Cx<float> *>(a.in);
-    auto gauge = static_cast<const Cx<float> *>(a.gauge);
-    auto xp = static_cast<const Cx<float> *>(a.xpay);
-    auto k = a.dagger ? k_dslash_mfma32_groups<NC, true> : k_dslash_mfma32_groups<NC, false>;
-    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g.Vh, a.parity, tab.walk, seg);
+    auto out = static_cast<Cx<T> *>(a.out);
+    auto in = static_cast<const Cx<T> *>(a.in);
+    auto gauge = static_cast<const Cx<T> *>(a.gauge);
+    auto xp = static_cast<const Cx<T> *>(a.xpay);
+    auto k = a.dagger ? k_dslash_mfma32_groups<T, NC, true> : k_dslash_mfma32_groups<T, NC, false>;
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, T(a.coef), a.g.Vh, a.parity, tab.walk, seg);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_prec(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_nc<T, 2>(a, t, stream);
+        case 3: return launch_nc<T, 3>(a, t, stream);
+        case 4: return launch_nc<T, 4>(a, t, stream);
+        case 8: return launch_nc<T, 8>(a, t, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_groups_prec(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_groups_nc<T, 2>(a, tab, by, bz, stream);
+        case 3: return launch_groups_nc<T, 3>(a, tab, by, bz, stream);
+        case 4: return launch_groups_nc<T, 4>(a, tab, by, bz, stream);
+        case 8: return launch_groups_nc<T, 8>(a, tab, by, bz, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -229,31 +270,24 @@ bool dslash_mfma32_supported(int Nc, int prec) {
     return prec == kSingle && (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8);
 }
 
+bool dslash_mfma64_supported(int Nc, int prec) {
+    return prec == kDouble && (Nc == 2 || Nc == 3 || Nc == 4 || Nc == 8);
+}
+
 hipError_t launch_dslash_mfma32(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
-    if (!dslash_mfma32_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
+    if (!dslash_mfma32_supported(a.Nc, a.prec) && !dslash_mfma64_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
     if (t.by <= 0 || t.bz <= 0 || t.spw <= 0 || a.g.L[1] % t.by || a.g.L[2] % t.bz) return hipErrorInvalidValue;
-    switch (a.Nc) {
-        case 2: return launch_nc<2>(a, t, stream);
-        case 3: return launch_nc<3>(a, t, stream);
-        case 4: return launch_nc<4>(a, t, stream);
-        case 8: return launch_nc<8>(a, t, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return a.prec == kDouble ? launch_prec<double>(a, t, stream) : launch_prec<float>(a, t, stream);
 }
 
 hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream) {
-    if (!dslash_mfma32_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
+    if (!dslash_mfma32_supported(a.Nc, a.prec) && !dslash_mfma64_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
     if (tab.G < 1 || tab.G > kMaxGroups || by <= 0 || bz <= 0 || a.g.L[1] % by || a.g.L[2] % bz)
         return hipErrorInvalidValue;
     for (int k = 0; k < tab.G; ++k)
         if (tab.spw[k] <= 0 || tab.R[k] < 1 || !dslash_supported(a.Nc, tab.R[k])) return hipErrorInvalidValue;
-    switch (a.Nc) {
-        case 2: return launch_groups_nc<2>(a, tab, by, bz, stream);
-        case 3: return launch_groups_nc<3>(a, tab, by, bz, stream);
-        case 4: return launch_groups_nc<4>(a, tab, by, bz, stream);
-        case 8: return launch_groups_nc<8>(a, tab, by, bz, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return a.prec == kDouble ? launch_groups_prec<double>(a, tab, by, bz, stream)
+                             : launch_groups_prec<float>(a, tab, by, bz, stream);
 }
 
 }  // namespace qcu

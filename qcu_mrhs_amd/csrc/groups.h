@@ -52,6 +52,8 @@ public:
     hipError_t autotune(const void *glo, int prec, hipStream_t s);
     // dslash kernel the autotuner chose for the batch (2 = VALU wave-per-site, 3 = FP32 matrix cores)
     int dslash_variant() const { return variant_; }
+    // and for the FP64 hops of the defect correction (2 = VALU, 3 = FP64 matrix cores; timed in solve)
+    int dslash_variant64() const { return variant64_; }
 
     // Solve M x = b for the batch held in b_full(); result in x_full().
     hipError_t solve(const void *g64, const void *glo, int inner_prec, double kappa, int max_iter, double tol,
@@ -62,6 +64,7 @@ private:
     hipError_t dslash(void *out, const void *in, const void *gauge, int prec, int parity, const void *xpay,
                       double coef, const SegTab &tab, hipStream_t s);
     hipError_t iteration(const void *glo, int P, double kappa, hipStream_t s);
+    hipError_t tune64(const void *g64, hipStream_t s);
     SegTab tab_with(const int *done) const;
 
     Geom g_;
@@ -71,6 +74,8 @@ private:
     SegTab tab_;
     int by_ = 0, bz_ = 0;
     int variant_ = 2;  // group dslash kernel: 2 = wave-per-site VALU, 3 = FP32 matrix cores (autotuned)
+    int variant64_ = 2;  // the same for the FP64 hops (2 = VALU, 3 = FP64 matrix cores)
+    bool tuned64_ = false;
     int lo_prec_ = -1;
     bool tuned_ = false;
     int tuned_prec_ = -1;

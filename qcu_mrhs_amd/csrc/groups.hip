@@ -96,9 +96,50 @@ hipError_t GroupSolver::ensure(int prec) {
 hipError_t GroupSolver::dslash(void *out, const void *in, const void *gauge, int prec, int parity,
                                const void *xpay, double coef, const SegTab &tab, hipStream_t s) {
     DslashArgs a{out, in, gauge, xpay, coef, g_, Nc_, 0, parity, 0, prec};
-    if (variant_ == kDslashMfma && dslash_mfma32_supported(Nc_, prec))
+    const int v = prec == kDouble ? variant64_ : variant_;
+    if (v == kDslashMfma && dslash_groups_matrix_supported(Nc_, prec))
         return launch_dslash_mfma32_groups(a, tab, by_, bz_, s);
     return launch_dslash_wave_groups(a, tab, by_, bz_, s);
+}
+
+// FP64 hops of the defect correction: the whole batch's hop timed on the VALU wave kernel and on
+// the FP64 matrix cores (each group with its inner-precision shape), the faster one kept
+hipError_t GroupSolver::tune64(const void *g64, hipStream_t s) {
+    if (tuned64_) return hipSuccess;
+    variant64_ = kDslashWaveSite;
+    if (!dslash_mfma64_supported(Nc_, kDouble)) {
+        tuned64_ = true;
+        return hipSuccess;
+    }
+    const size_t bytes = size_t(n_) * tab_.Rtot * sizeof(C64);
+    QCU_HIP_TRY(hipMemsetAsync(t64_, 0, bytes, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QCU_HIP_TRY(hipEventCreate(&e0));
+    QCU_HIP_TRY(hipEventCreate(&e1));
+    const SegTab all = tab_with(nullptr);
+    hipError_t err = hipSuccess;
+    float best = 1e30f;
+    int best_v = kDslashWaveSite;
+    for (int v : {int(kDslashWaveSite), int(kDslashMfma)}) {
+        variant64_ = v;
+        if (err == hipSuccess) err = dslash(u64_, t64_, g64, kDouble, 0, nullptr, 1.0, all, s);  // warm-up
+        if (err == hipSuccess) err = hipEventRecord(e0, s);
+        for (int rep = 0; rep < 3 && err == hipSuccess; ++rep)
+            err = dslash(u64_, t64_, g64, kDouble, 0, nullptr, 1.0, all, s);
+        if (err == hipSuccess) err = hipEventRecord(e1, s);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err == hipSuccess && ms < best) {
+            best = ms;
+            best_v = v;
+        }
+    }
+    variant64_ = best_v;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err == hipSuccess) tuned64_ = true;
+    return err;
 }
 
 hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
@@ -116,7 +157,7 @@ hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
     // wavefront, then the whole batch with those shapes; the variant whose batch launch is
     // faster serves all groups (one launch covers the batch).
     std::vector<int> variants{kDslashWaveSite};
-    if (dslash_mfma32_supported(Nc_, prec)) variants.push_back(kDslashMfma);
+    if (dslash_groups_matrix_supported(Nc_, prec)) variants.push_back(kDslashMfma);
     double best_total = 1e30;
     int best_variant = kDslashWaveSite;
     int best_spw_of[kMaxGroups] = {};
@@ -204,6 +245,7 @@ hipError_t GroupSolver::solve(const void *g64, const void *glo, int inner_prec, 
     const int P = inner_prec;
     QCU_HIP_TRY(ensure(P));
     QCU_HIP_TRY(autotune(glo, P, s));
+    QCU_HIP_TRY(tune64(g64, s));
     const auto t0 = std::chrono::steady_clock::now();
     const int G = tab_.G, Rt = tab_.Rtot;
     const size_t nR = size_t(n_) * Rt;

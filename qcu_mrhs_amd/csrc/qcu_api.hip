@@ -853,6 +853,12 @@ int qcuGroupsDslashVariant(void) {
     return c->groups ? c->groups->dslash_variant() : 0;
 }
 
+int qcuGroupsDslashVariant64(void) {
+    Context *c = ctx_or_error();
+    if (!c) return -1;
+    return c->groups ? c->groups->dslash_variant64() : 0;
+}
+
 // ---- learned preconditioner --------------------------------------------
 int qcuLgnnSetWeights(int channels, const double *w1, const double *w2) {
     Context *c = ctx_or_error();

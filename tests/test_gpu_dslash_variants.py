@@ -99,30 +99,31 @@ def test_mfma_persistent_grid_production_shape(prec, spw):
         assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec]
 
 
+@pytest.mark.parametrize("prec", [S, D])
 @pytest.mark.parametrize("L,Nc,R", [((4, 4, 4, 4), 8, 18), ((4, 4, 4, 8), 4, 5), ((6, 4, 4, 4), 3, 17),
                                     ((4, 4, 4, 8), 2, 3), ((16, 16, 16, 16), 8, 16)])
-def test_fp32_matrix_core_dslash_matches_oracle(L, Nc, R):
-    """FP32 storage on the FP32 matrix cores (v_mfma_f32_16x16x4_f32): both
-    parities, dagger, the xpay form, one and several sites per wavefront,
-    RHS counts with partial 16-RHS chunks, against the FP64 oracle at the
-    FP32 storage bound."""
-    print("case mfma32", L, Nc, R, flush=True)
+def test_fp32_fp64_matrix_core_dslash_matches_oracle(L, Nc, R, prec):
+    """FP32 storage on the FP32 matrix cores (v_mfma_f32_16x16x4_f32) and FP64
+    storage on the FP64 ones (v_mfma_f64_16x16x4_f64): both parities, dagger,
+    the xpay form, one and several sites per wavefront, RHS counts with partial
+    16-RHS chunks, against the FP64 oracle at the storage precision's bound."""
+    print("case mfma32/64", L, Nc, R, prec, flush=True)
     gauge = fields.gauge_field(L, Nc, 500 + Nc + R, kind="haar")
     src = fields.gaussian_sources(L, Nc, R, 600 + R).reshape(R, 2, -1)
-    d_g = api.DeviceArray.from_complex(gauge.reshape(-1), S)
+    d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
     for spw in (1, 4):
         api.dslash_tune("mfma", 0, 0, spw)
         for parity, dagger in ((1, 0), (0, 1)):
             inp = fields.to_mrhs(src[:, 1 - parity], Nc)
             xp = fields.to_mrhs(src[:, parity], Nc)
-            d_in = api.DeviceArray.from_complex(inp, S)
-            d_xp = api.DeviceArray.from_complex(xp, S)
-            d_out = api.DeviceArray(api.storage_bytes(inp.size, S))
+            d_in = api.DeviceArray.from_complex(inp, prec)
+            d_xp = api.DeviceArray.from_complex(xp, prec)
+            d_out = api.DeviceArray(api.storage_bytes(inp.size, prec))
             ref = oracle.dslash(inp, gauge, L, Nc, R, parity, dagger)
-            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=S)
-            assert _rel(d_out.complex(inp.size, S), ref) < TOL[S], (spw, parity, dagger)
-            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=S, xpay=d_xp, coef=-0.13)
-            assert _rel(d_out.complex(inp.size, S), xp - 0.13 * ref) < TOL[S], (spw, "xpay")
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec)
+            assert _rel(d_out.complex(inp.size, prec), ref) < TOL[prec], (spw, parity, dagger)
+            api.dslash_mrhs(d_out, d_in, d_g, L, Nc, R, parity, dagger, precision=prec, xpay=d_xp, coef=-0.13)
+            assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec], (spw, "xpay")
 
 
 @pytest.mark.parametrize("L,Nc,R,ts", [((8, 4, 4, 6), 3, 17, 4), ((4, 6, 4, 8), 2, 3, 5), ((4, 4, 6, 4), 4, 5, 1),

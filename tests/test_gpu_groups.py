@@ -31,7 +31,8 @@ def _run(L, Nc, kappa, sizes, seed, inner=1, tol=1e-10, user_prec=2):
         groups.append((xs, bs))
     infos = api.invert_groups(groups, 3000, tol)
     print("group dslash kernel:", {2: "VALU wave per site", 3: "FP32 matrix cores"}.get(
-        api.lib().qcuGroupsDslashVariant(), "?"), flush=True)
+        api.lib().qcuGroupsDslashVariant(), "?"), "; FP64 hops:", {2: "VALU wave per site", 3: "FP64 matrix cores"}.get(
+        api.lib().qcuGroupsDslashVariant64(), "?"), flush=True)
     xs = [np.stack([x.complex(srcs[k].shape[1], user_prec) for x in groups[k][0]]) for k in range(len(sizes))]
     api.finalizeQcu()
     return gauge, srcs, xs, infos
