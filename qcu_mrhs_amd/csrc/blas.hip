@@ -556,7 +556,27 @@ __global__ __launch_bounds__(256) void k_accum_seg(Cx<double> *xhi, const CS *xl
     const long N = tab.n * R;
     xhi += off;
     xlo += off;
-    for (long e = g; e < N; e += stride) {
+    // four entries per pass, every load issued before the first use (one entry per pass leaves a
+    // wavefront with too few bytes in flight to stream)
+    constexpr int U = 4;
+    long e = g;
+    for (; e + (U - 1) * stride < N; e += U * stride) {
+        CS v[U];
+        Cx<double> h[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            v[k] = xlo[e + k * stride];
+            h[k] = xhi[e + k * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const Cx<double> w = load_cx<double>(&v[k]);
+            h[k].re = fma(sc, w.re, h[k].re);
+            h[k].im = fma(sc, w.im, h[k].im);
+            xhi[e + k * stride] = h[k];
+        }
+    }
+    for (; e < N; e += stride) {
         const Cx<double> v = load_cx<double>(xlo + e);
         Cx<double> h = xhi[e];
         h.re = fma(sc, v.re, h.re);
