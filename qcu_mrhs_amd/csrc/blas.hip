@@ -556,8 +556,7 @@ __global__ __launch_bounds__(256) void k_accum_seg(Cx<double> *xhi, const CS *xl
     const long N = tab.n * R;
     xhi += off;
     xlo += off;
-    // four entries per pass, every load issued before the first use (one entry per pass leaves a
-    // wavefront with too few bytes in flight to stream)
+    // four entries per pass, every load issued before the first use
     constexpr int U = 4;
     long e = g;
     for (; e + (U - 1) * stride < N; e += U * stride) {
