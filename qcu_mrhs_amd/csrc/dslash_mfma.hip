@@ -513,15 +513,17 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
 // 8 neighbour spinors per site); small workgroups let several share a CU, so
 // one workgroup's memory wait overlaps another's arithmetic.
 //
-// Measured at 24^3 x 48, SU(3), 16 RHS, FP16 (Ts = 12): 24 % fewer L2 requests
-// per site than k_dslash_mfma, yet about the same time -- 228 us on the
-// autotuner's synthetic fill, 249-262 us on smooth links and Gaussian sources,
-// against 236-250 us for k_dslash_mfma.  The per-layer step chains an LDS read
-// phase, a link round trip and a workgroup barrier on the staged loads, and 5
-// workgroups per CU (LDS and registers) do not hide that chain as well as the
-// free-running wavefronts of k_dslash_mfma hide their L1 misses.  So this form
-// is selected explicitly (qcuDslashTune / qcuDslashSetTuned / a tuning file),
-// not by the autotuner.
+// Measured at 24^3 x 48, SU(3), 16 RHS, FP16 (Ts = 12): 20 % fewer L2 requests
+// per site than k_dslash_mfma (39.7 against 49.4) but a lower L2 hit rate (43
+// against 69 %) with workgroups ordered row by row; ordered in (y, z) blocks,
+// 232-235 us against 236-254 us row by row and 238-250 us for k_dslash_mfma in
+// the same processes -- within the spread between GPUs.  The per-layer step
+// chains an LDS read phase, a link round trip and a workgroup barrier on the
+// staged loads, which 5 workgroups per CU hide about as well as the free-running
+// wavefronts of k_dslash_mfma hide their L1 misses; and the solver's Schur
+// operator would lose k_dslash_mfma's fused omega reduction.  So this form is
+// selected explicitly (qcuDslashTune / qcuDslashSetTuned / a tuning file), not
+// by the autotuner.
 constexpr int kLdsSites = 4;
 
 using lds_void = __attribute__((address_space(3))) void;
@@ -562,8 +564,8 @@ __global__ __launch_bounds__(64 * kLdsSites) void k_dslash_lds(uint32_t *__restr
                                                               const uint32_t *__restrict__ in,
                                                               const uint32_t *__restrict__ gauge,
                                                               const uint32_t *__restrict__ xpay, float coef, Geom g,
-                                                              int R, int parity, int Ts, unsigned vec_bytes,
-                                                              unsigned gauge_bytes) {
+                                                              int R, int parity, int Ts, int by, int bz,
+                                                              unsigned vec_bytes, unsigned gauge_bytes) {
     static_assert(NC <= 4, "one k step per four hops");
     using P = Pk<S>;
     using PH = PkPhase<S>;
@@ -596,14 +598,22 @@ __global__ __launch_bounds__(64 * kLdsSites) void k_dslash_lds(uint32_t *__restr
     const auto r_out = rsrc(out, vec_bytes);
     const auto r_x = rsrc(xpay, xpay ? vec_bytes : 0u);
 
-    // workgroup -> (x segment, y, z, t run): consecutive workgroups on one XCD take adjacent segments
-    // and rows, which share their side and halo rows in its L2
+    // workgroup -> (x segment, y, z, t run): consecutive workgroups on one XCD take the segments of the
+    // rows of a by x bz block of the (y, z) plane, then the next block, so the workgroups an XCD runs at
+    // once stage each other's rows as side rows while those are in its L2
     const int nxs = (Lxh + SEG - 1) / SEG;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int xs = b % nxs;
-    const int y = (b / nxs) % Ly;
-    const int z = (b / (nxs * Ly)) % Lz;
-    const int seg = b / (nxs * Ly * Lz);
+    int rest = b / nxs;
+    const int iy = rest % by;
+    rest /= by;
+    const int iz = rest % bz;
+    rest /= bz;
+    const int nty = Ly / by, ntz = Lz / bz;
+    const int y = (rest % nty) * by + iy;
+    rest /= nty;
+    const int z = (rest % ntz) * bz + iz;
+    const int seg = rest / ntz;
     const int a0 = xs * SEG;
     const int t0 = seg * Ts, t1 = min(t0 + Ts, Lt);
     const int yp = y + 1 == Ly ? 0 : y + 1, ym = y == 0 ? Ly - 1 : y - 1;
@@ -840,7 +850,10 @@ hipError_t launch_lds(const DslashArgs &a, const DslashTune &t, hipStream_t stre
     const unsigned gauge_bytes = unsigned(size_t(a.g.Vh) * 8 * a.Nc * a.Nc * 4);
     auto k = a.dagger ? (xp ? k_dslash_lds<S, NC, true, true> : k_dslash_lds<S, NC, true, false>)
                       : (xp ? k_dslash_lds<S, NC, false, true> : k_dslash_lds<S, NC, false, false>);
-    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g, a.R, a.parity, Ts,
+    // (y, z) block of the workgroup order: by | Ly, bz | Lz (else rows one by one)
+    const int by = t.by > 0 && a.g.L[1] % t.by == 0 ? t.by : 1;
+    const int bz = t.bz > 0 && a.g.L[2] % t.bz == 0 ? t.bz : 1;
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, float(a.coef), a.g, a.R, a.parity, Ts, by, bz,
                        vec_bytes, gauge_bytes);
     return hipGetLastError();
 }
