@@ -265,6 +265,9 @@ def bench_groups(args):
            "data": "synthetic (Haar SU(N) links, Gaussian sources)", "dtype": "f32 storage inner, f64 outer",
            "config": {"workload": "SU(N) Wilson 16^4, groups of 3, 8, 17, 32 sources, kappa 0.12, tol 1e-10",
                       "lattice": list(L), "groups": list(sizes)}, "results": {}}
+    gk = [int(v) for v in args.groups_kernels.split(",")]
+    api.check(api.lib().qcuGroupsDslashTune(gk[0], gk[1]))
+    out["config"]["kernels"] = args.groups_kernels
     for Nc in (4, 8):
         gauge = fields.gauge_field(L, Nc, 5000 + Nc, kind="haar")
         api.initGridSize(L, Nc, 4, 2, 1)
@@ -286,8 +289,8 @@ def bench_groups(args):
             runs.append((time.perf_counter() - t0, info))
         wall = float(np.mean([r[0] for r in runs]))
         info = runs[-1][1]
-        variant = {2: "VALU wave per site", 3: "FP32 matrix cores"}.get(api.lib().qcuGroupsDslashVariant(), "?")
-        variant64 = {2: "VALU wave per site", 3: "FP64 matrix cores"}.get(api.lib().qcuGroupsDslashVariant64(), "?")
+        variant = {2: "VALU wave per site", 3: "FP32 matrix cores", 5: "FP32 matrix cores, packed RHS"}.get(api.lib().qcuGroupsDslashVariant(), "?")
+        variant64 = {2: "VALU wave per site", 3: "FP64 matrix cores", 5: "FP64 matrix cores, packed RHS"}.get(api.lib().qcuGroupsDslashVariant64(), "?")
         res = {"batch_s": wall, "rhs_per_s": sum(sizes) / wall, "dslash_kernel": variant,
                "dslash_kernel_fp64": variant64,
                "groups": [{"m_rhs": R, "converged": bool(i["converged"]), "iterations": i["iterations"],
@@ -340,6 +343,9 @@ def main():
                          "else written after autotuning (so a profiled run can reuse the shapes)")
     ap.add_argument("--dslash-precs", default="double,single,half,bf16",
                     help="storage precisions timed by the dslash measurement")
+    ap.add_argument("--groups-kernels", default="0,0",
+                    help="with --groups: dslash kernels of the inner and the FP64 hops (0 = timed choice, 2 = VALU, "
+                         "3 = matrix cores per group, 5 = matrix cores over packed RHS chunks)")
     ap.add_argument("--groups", action="store_true",
                     help="batched uneven RHS groups instead: SU(4) and SU(8) on 16^4, groups of 3, 8, 17 "
                          "and 32 sources, FP32 inner, tol 1e-10 (prints its own JSON line)")

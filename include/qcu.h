@@ -213,13 +213,21 @@ typedef struct QcuGroupInfo_s {
 int qcuInvertGroups(int n_groups, const QcuRhsGroup *groups, int max_iteration, double tol, QcuGroupInfo *info);
 /* Dslash kernel the last qcuInvertGroups batch ran, as chosen by its timing
  * of the candidates: 2 = VALU wave per site, 3 = FP32 matrix cores
- * (v_mfma_f32_16x16x4_f32, FP32 storage); 0 before any batch, -1 without
- * initGridSize. */
+ * (v_mfma_f32_16x16x4_f32, FP32 storage) per group, 5 = FP32 matrix cores
+ * over the groups' right-hand sides packed into shared 16-RHS chunks; 0
+ * before any batch, -1 without initGridSize. */
 int qcuGroupsDslashVariant(void);
 /* The same for the FP64 hops of the batch's defect correction: 2 = VALU wave
- * per site, 3 = FP64 matrix cores (v_mfma_f64_16x16x4_f64); timed once per
- * batch shape.  No reference counterpart. */
+ * per site, 3 / 5 = FP64 matrix cores (v_mfma_f64_16x16x4_f64) per group /
+ * over packed chunks; timed once per batch shape.  No reference counterpart. */
 int qcuGroupsDslashVariant64(void);
+/* Choose the batched groups' dslash kernels for every later qcuInvertGroups
+ * call in this process (after each batch shape's own timing): variant for the
+ * inner hops, variant64 for the FP64 hops; 0 = the timed choice, 2 = VALU wave
+ * per site, 3 = matrix cores per group, 5 = matrix cores over packed RHS
+ * chunks (3 and 5 need FP32 / FP64 storage; otherwise the timed choice
+ * stays).  No reference counterpart. */
+int qcuGroupsDslashTune(int variant, int variant64);
 
 /* Device-memory plumbing for callers that have no other GPU runtime. */
 void *qcuDeviceMalloc(size_t bytes);

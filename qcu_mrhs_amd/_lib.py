@@ -117,6 +117,7 @@ def load() -> ctypes.CDLL:
         "qcuInvertGroups": ([i, ctypes.POINTER(QcuRhsGroup), i, d, ctypes.POINTER(QcuGroupInfo)], i),
         "qcuGroupsDslashVariant": ([], i),
         "qcuGroupsDslashVariant64": ([], i),
+        "qcuGroupsDslashTune": ([i, i], i),
         "qcuLgnnSetWeights": ([i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnGetWeights": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnApply": ([vp, vp, i], i),

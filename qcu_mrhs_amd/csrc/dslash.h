@@ -66,6 +66,9 @@ bool dslash_lds_supported(const Geom &g, int Nc, int prec);
 // FP32 / FP64 storage on the FP32 / FP64 matrix cores (dslash_mfma32.hip), also for batched groups
 hipError_t launch_dslash_mfma32(const DslashArgs &a, const DslashTune &t, hipStream_t stream);
 hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, int by, int bz, hipStream_t stream);
+// the groups' RHS packed into shared 16-RHS chunks (groups back to back: base[k] = base[k-1] + R[k-1])
+hipError_t launch_dslash_mfma32_packed(const DslashArgs &a, const SegTab &tab, int by, int bz, int spw,
+                                       hipStream_t stream);
 bool dslash_mfma32_supported(int Nc, int prec);
 bool dslash_mfma64_supported(int Nc, int prec);
 // a matrix-core kernel exists for (Nc, prec)

@@ -116,16 +116,9 @@ __device__ __forceinline__ void mfma32_hop(typename Acc<T>::type (&acc)[4], cons
     }
 }
 
-// Sites q of the traversal handled by this wavefront: spw consecutive positions starting at
-// (b * 4 + wavefront) * spw; every chunk of 16 RHS of each site.
-template <typename T, int NC, bool DAG>
-__device__ __forceinline__ void mfma32_block(int b, Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
-                                             const Cx<T> *__restrict__ gauge, const Cx<T> *__restrict__ xpay,
-                                             T coef, int Vh, int R, int parity, const int *__restrict__ walk,
-                                             int spw) {
-    using V = typename Acc<T>::type;
+template <typename T, int NC>
+__device__ __forceinline__ Mfma32Lane<NC> mfma32_lane() {
     const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     Mfma32Lane<NC> ln;
     ln.m = lane & 15;
     ln.g = lane >> 4;
@@ -141,6 +134,19 @@ __device__ __forceinline__ void mfma32_block(int b, Cx<T> *__restrict__ out, con
     }
     ln.arow = ln.c < NC;
     if (!ln.arow) ln.c = NC - 1;  // clamped address, A operand zeroed
+    return ln;
+}
+
+// Sites q of the traversal handled by this wavefront: spw consecutive positions starting at
+// (b * 4 + wavefront) * spw; every chunk of 16 RHS of each site.
+template <typename T, int NC, bool DAG>
+__device__ __forceinline__ void mfma32_block(int b, Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
+                                             const Cx<T> *__restrict__ gauge, const Cx<T> *__restrict__ xpay,
+                                             T coef, int Vh, int R, int parity, const int *__restrict__ walk,
+                                             int spw) {
+    using V = typename Acc<T>::type;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Mfma32Lane<NC> ln = mfma32_lane<T, NC>();
     const long site_len = 4L * NC * R;
     const long link_len = long(NC) * NC;
     const int *wtab = walk + long(parity) * Vh * 9;
@@ -211,6 +217,76 @@ __global__ __launch_bounds__(256) void k_dslash_mfma32_groups(Cx<T> *__restrict_
                              coef, Vh, tab.R[grp], parity, walk, spw);
 }
 
+// Every group of a batch in one launch with the groups' RHS packed into shared 16-RHS chunks
+// (blockIdx.y = chunk): lane m of chunk k takes batch RHS q = 16 k + m, i.e. RHS q - base[g] of the
+// group g holding q, at its own offset and stride in the group-concatenated [n][R_g] vectors.  Groups
+// of 3, 8, 17 and 32 sources then fill 4 chunks per site instead of 6 (1 + 1 + 2 + 2), and the links
+// and matrix-core work per site shrink with them.  Lanes of a finished group compute and store into
+// its work vector (never read again); a chunk whose groups have all finished is skipped.
+template <typename T, int NC, bool DAG>
+__global__ __launch_bounds__(256) void k_dslash_mfma32_packed(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ in,
+                                                             const Cx<T> *__restrict__ gauge,
+                                                             const Cx<T> *__restrict__ xpay, T coef, int Vh,
+                                                             int parity, const int *__restrict__ walk, SegTab tab,
+                                                             int spw) {
+    using V = typename Acc<T>::type;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const Mfma32Lane<NC> ln = mfma32_lane<T, NC>();
+    // this lane's batch RHS and its group
+    const int qr = 16 * int(blockIdx.y) + ln.m;
+    const bool valid = qr < tab.Rtot;
+    const int q = valid ? qr : tab.Rtot - 1;
+    int grp = 0;
+#pragma unroll
+    for (int k = 1; k < kMaxGroups; ++k)
+        if (k < tab.G && tab.base[k] <= q) grp = k;
+    const bool active = !(tab.done && tab.done[grp]);
+    if (!__any(valid && active)) return;  // wavefront-uniform
+    const long Rg = tab.R[grp];
+    const long lane_off = tab.n * tab.base[grp] + (q - tab.base[grp]);  // complex entries
+    const long site_len = 4L * NC * Rg;  // of this lane's group
+    const long link_len = long(NC) * NC;
+    const int *wtab = walk + long(parity) * Vh * 9;
+    const int q0 = (xcd_remap(blockIdx.x, gridDim.x) * 4 + wv) * spw;
+    const Cx<T> *inl = in + lane_off;
+    for (int j = 0; j < spw; ++j) {
+        const int pos = q0 + j;
+        if (pos >= Vh) break;  // uniform
+        int nbr[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) nbr[k] = __builtin_amdgcn_readfirstlane(wtab[long(pos) * 9 + k]);
+        const int site = nbr[0];
+        V acc[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc[s] = V{T(0), T(0), T(0), T(0)};
+#define QCU_HOP(MU)                                                                                              \
+    mfma32_hop<T, NC, DAG, MU, true>(acc, inl + nbr[1 + 2 * MU] * site_len,                                      \
+                                     gauge + ((MU * 2 + parity) * long(Vh) + site) * link_len, ln, Rg, 0);      \
+    mfma32_hop<T, NC, DAG, MU, false>(acc, inl + nbr[2 + 2 * MU] * site_len,                                     \
+                                      gauge + ((MU * 2 + 1 - parity) * long(Vh) + nbr[2 + 2 * MU]) * link_len, ln, Rg, 0);
+        QCU_HOP(0)
+        QCU_HOP(1)
+        QCU_HOP(2)
+        QCU_HOP(3)
+#undef QCU_HOP
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int c = 2 * ln.g + e;
+                if (c < NC) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const long o = lane_off + site * site_len + (long(s) * NC + c) * Rg;
+                        Cx<T> v = {coef * acc[s][2 * e], coef * acc[s][2 * e + 1]};
+                        if (xpay) v = v + xpay[o];
+                        out[o] = v;
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <typename T, int NC>
 hipError_t launch_nc(const DslashArgs &a, const DslashTune &t, hipStream_t stream) {
     GeomTables tab;
@@ -240,6 +316,32 @@ hipError_t launch_groups_nc(const DslashArgs &a, const SegTab &seg, int by, int 
     auto k = a.dagger ? k_dslash_mfma32_groups<T, NC, true> : k_dslash_mfma32_groups<T, NC, false>;
     hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, T(a.coef), a.g.Vh, a.parity, tab.walk, seg);
     return hipGetLastError();
+}
+
+template <typename T, int NC>
+hipError_t launch_packed_nc(const DslashArgs &a, const SegTab &seg, int by, int bz, int spw, hipStream_t stream) {
+    GeomTables tab;
+    QCU_HIP_TRY(geom_tables(a.g, by, bz, tab, stream));
+    dim3 grid(div_up(a.g.Vh, 4 * spw), div_up(seg.Rtot, 16)), block(256);
+    auto out = static_cast<Cx<T> *>(a.out);
+    auto in = static_cast<const Cx<T> *>(a.in);
+    auto gauge = static_cast<const Cx<T> *>(a.gauge);
+    auto xp = static_cast<const Cx<T> *>(a.xpay);
+    auto k = a.dagger ? k_dslash_mfma32_packed<T, NC, true> : k_dslash_mfma32_packed<T, NC, false>;
+    hipLaunchKernelGGL(k, grid, block, 0, stream, out, in, gauge, xp, T(a.coef), a.g.Vh, a.parity, tab.walk, seg,
+                       spw);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_packed_prec(const DslashArgs &a, const SegTab &tab, int by, int bz, int spw, hipStream_t stream) {
+    switch (a.Nc) {
+        case 2: return launch_packed_nc<T, 2>(a, tab, by, bz, spw, stream);
+        case 3: return launch_packed_nc<T, 3>(a, tab, by, bz, spw, stream);
+        case 4: return launch_packed_nc<T, 4>(a, tab, by, bz, spw, stream);
+        case 8: return launch_packed_nc<T, 8>(a, tab, by, bz, spw, stream);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>
@@ -288,6 +390,19 @@ hipError_t launch_dslash_mfma32_groups(const DslashArgs &a, const SegTab &tab, i
         if (tab.spw[k] <= 0 || tab.R[k] < 1 || !dslash_supported(a.Nc, tab.R[k])) return hipErrorInvalidValue;
     return a.prec == kDouble ? launch_groups_prec<double>(a, tab, by, bz, stream)
                              : launch_groups_prec<float>(a, tab, by, bz, stream);
+}
+
+hipError_t launch_dslash_mfma32_packed(const DslashArgs &a, const SegTab &tab, int by, int bz, int spw,
+                                       hipStream_t stream) {
+    if (!dslash_mfma32_supported(a.Nc, a.prec) && !dslash_mfma64_supported(a.Nc, a.prec)) return hipErrorInvalidValue;
+    if (tab.G < 1 || tab.G > kMaxGroups || tab.Rtot < 1 || spw <= 0 || by <= 0 || bz <= 0 || a.g.L[1] % by ||
+        a.g.L[2] % bz)
+        return hipErrorInvalidValue;
+    for (int k = 0; k < tab.G; ++k)
+        if (tab.R[k] < 1 || !dslash_supported(a.Nc, tab.R[k]) || (k > 0 && tab.base[k] != tab.base[k - 1] + tab.R[k - 1]))
+            return hipErrorInvalidValue;
+    return a.prec == kDouble ? launch_packed_prec<double>(a, tab, by, bz, spw, stream)
+                             : launch_packed_prec<float>(a, tab, by, bz, spw, stream);
 }
 
 }  // namespace qcu

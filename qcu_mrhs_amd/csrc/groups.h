@@ -32,6 +32,13 @@ struct GroupResult {
     double seconds = 0.0;        // from the start of the solve to the group's convergence
 };
 
+// groups dslash variant: matrix cores over the groups' RHS packed into shared 16-RHS chunks
+constexpr int kGroupsPacked = 5;
+
+// Process-wide override of the groups' dslash kernels (inner hops, FP64 hops): 0 = the timed choice,
+// else 2 (VALU), 3 (matrix cores per group) or 5 (packed); applied after the timing of each batch shape.
+void groups_force_variants(int inner, int fp64);
+
 class GroupSolver {
 public:
     GroupSolver(const Geom &g, int Nc, const std::vector<int> &Rs);
@@ -50,9 +57,10 @@ public:
     // Allocates the batch buffers and picks each group's dslash shape (sites
     // per wavefront) by timing it on the inner precision; cached per precision.
     hipError_t autotune(const void *glo, int prec, hipStream_t s);
-    // dslash kernel the autotuner chose for the batch (2 = VALU wave-per-site, 3 = FP32 matrix cores)
+    // dslash kernel the autotuner chose for the batch (2 = VALU wave-per-site, 3 = FP32 matrix cores per
+    // group, 5 = FP32 matrix cores over the groups' RHS packed into shared chunks)
     int dslash_variant() const { return variant_; }
-    // and for the FP64 hops of the defect correction (2 = VALU, 3 = FP64 matrix cores; timed in solve)
+    // and for the FP64 hops of the defect correction (2 = VALU, 3 / 5 = FP64 matrix cores; timed in solve)
     int dslash_variant64() const { return variant64_; }
 
     // Solve M x = b for the batch held in b_full(); result in x_full().
@@ -73,8 +81,12 @@ private:
     long n_;
     SegTab tab_;
     int by_ = 0, bz_ = 0;
-    int variant_ = 2;  // group dslash kernel: 2 = wave-per-site VALU, 3 = FP32 matrix cores (autotuned)
-    int variant64_ = 2;  // the same for the FP64 hops (2 = VALU, 3 = FP64 matrix cores)
+    // group dslash kernel (autotuned): 2 = wave-per-site VALU, 3 = matrix cores per group, 5 = matrix cores
+    // with the groups' RHS packed into shared 16-RHS chunks (sites per wavefront spw_packed_)
+    int variant_ = 2;
+    int spw_packed_ = 1;
+    int variant64_ = 2;  // the same for the FP64 hops
+    int spw_packed64_ = 1;
     bool tuned64_ = false;
     int lo_prec_ = -1;
     bool tuned_ = false;

@@ -93,10 +93,21 @@ hipError_t GroupSolver::ensure(int prec) {
     return hipSuccess;
 }
 
+namespace {
+int g_force_inner = 0, g_force_fp64 = 0;
+}  // namespace
+
+void groups_force_variants(int inner, int fp64) {
+    g_force_inner = inner;
+    g_force_fp64 = fp64;
+}
+
 hipError_t GroupSolver::dslash(void *out, const void *in, const void *gauge, int prec, int parity,
                                const void *xpay, double coef, const SegTab &tab, hipStream_t s) {
     DslashArgs a{out, in, gauge, xpay, coef, g_, Nc_, 0, parity, 0, prec};
     const int v = prec == kDouble ? variant64_ : variant_;
+    if (v == kGroupsPacked && dslash_groups_matrix_supported(Nc_, prec))
+        return launch_dslash_mfma32_packed(a, tab, by_, bz_, prec == kDouble ? spw_packed64_ : spw_packed_, s);
     if (v == kDslashMfma && dslash_groups_matrix_supported(Nc_, prec))
         return launch_dslash_mfma32_groups(a, tab, by_, bz_, s);
     return launch_dslash_wave_groups(a, tab, by_, bz_, s);
@@ -120,8 +131,12 @@ hipError_t GroupSolver::tune64(const void *g64, hipStream_t s) {
     hipError_t err = hipSuccess;
     float best = 1e30f;
     int best_v = kDslashWaveSite;
-    for (int v : {int(kDslashWaveSite), int(kDslashMfma)}) {
-        variant64_ = v;
+    int best_spw = 1;
+    const std::pair<int, int> cands[] = {{kDslashWaveSite, 0}, {kDslashMfma, 0}, {kGroupsPacked, 1},
+                                         {kGroupsPacked, 2}, {kGroupsPacked, 4}};
+    for (const auto &c : cands) {
+        variant64_ = c.first;
+        if (c.second > 0) spw_packed64_ = c.second;
         if (err == hipSuccess) err = dslash(u64_, t64_, g64, kDouble, 0, nullptr, 1.0, all, s);  // warm-up
         if (err == hipSuccess) err = hipEventRecord(e0, s);
         for (int rep = 0; rep < 3 && err == hipSuccess; ++rep)
@@ -132,10 +147,14 @@ hipError_t GroupSolver::tune64(const void *g64, hipStream_t s) {
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
         if (err == hipSuccess && ms < best) {
             best = ms;
-            best_v = v;
+            best_v = c.first;
+            best_spw = c.second;
         }
     }
     variant64_ = best_v;
+    if (best_spw > 0) spw_packed64_ = best_spw;
+    if (g_force_fp64 == kDslashWaveSite || g_force_fp64 == kDslashMfma || g_force_fp64 == kGroupsPacked)
+        variant64_ = g_force_fp64;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (err == hipSuccess) tuned64_ = true;
@@ -159,7 +178,7 @@ hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
     std::vector<int> variants{kDslashWaveSite};
     if (dslash_groups_matrix_supported(Nc_, prec)) variants.push_back(kDslashMfma);
     double best_total = 1e30;
-    int best_variant = kDslashWaveSite;
+    int best_variant = kDslashWaveSite, best_packed = 1;
     int best_spw_of[kMaxGroups] = {};
     for (int v : variants) {
         variant_ = v;
@@ -213,7 +232,34 @@ hipError_t GroupSolver::autotune(const void *glo, int prec, hipStream_t s) {
             for (int k = 0; k < tab_.G; ++k) best_spw_of[k] = spw_of[k];
         }
     }
+    // the groups' RHS packed into shared 16-RHS chunks: the whole batch over sites per wavefront
+    if (err == hipSuccess && dslash_groups_matrix_supported(Nc_, prec)) {
+        for (int j = 0; j < kMaxGroups; ++j) reinterpret_cast<int *>(h_)[j] = 0;
+        err = hipMemcpyAsync(done_, h_, sizeof(int) * kMaxGroups, hipMemcpyHostToDevice, s);
+        const SegTab t = tab_with(done_);
+        variant_ = kGroupsPacked;
+        for (int spw : {1, 2, 4, 8}) {
+            spw_packed_ = spw;
+            if (err == hipSuccess) err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);  // warm-up
+            if (err == hipSuccess) err = hipEventRecord(e0, s);
+            for (int rep = 0; rep < 5 && err == hipSuccess; ++rep)
+                err = dslash(tmp_, p_, glo, prec, 0, nullptr, 1.0, t, s);
+            if (err == hipSuccess) err = hipEventRecord(e1, s);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            if (err == hipSuccess && ms < best_total) {
+                best_total = ms;
+                best_variant = kGroupsPacked;
+                best_packed = spw;
+            }
+        }
+    }
     variant_ = best_variant;
+    spw_packed_ = best_packed;
+    if (g_force_inner == kDslashWaveSite || ((g_force_inner == kDslashMfma || g_force_inner == kGroupsPacked) &&
+                                             dslash_groups_matrix_supported(Nc_, prec)))
+        variant_ = g_force_inner;
     for (int k = 0; k < tab_.G; ++k) tab_.spw[k] = best_spw_of[k];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

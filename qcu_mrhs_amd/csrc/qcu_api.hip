@@ -853,6 +853,16 @@ int qcuGroupsDslashVariant(void) {
     return c->groups ? c->groups->dslash_variant() : 0;
 }
 
+int qcuGroupsDslashTune(int variant, int variant64) {
+    auto ok = [](int v) { return v == 0 || v == kDslashWaveSite || v == kDslashMfma || v == kGroupsPacked; };
+    if (!ok(variant) || !ok(variant64)) {
+        set_error(kQcuErrorInvalidValue, "qcuGroupsDslashTune: variants are 0, 2, 3 or 5");
+        return g_err;
+    }
+    groups_force_variants(variant, variant64);
+    return 0;
+}
+
 int qcuGroupsDslashVariant64(void) {
     Context *c = ctx_or_error();
     if (!c) return -1;

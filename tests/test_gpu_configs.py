@@ -186,8 +186,8 @@ def test_c4_groups_true_residual_per_group(Nc):
         groups.append(([api.DeviceArray(b[r].nbytes) for r in range(R)],
                        [api.DeviceArray.from_complex(b[r]) for r in range(R)]))
     infos = api.invert_groups(groups, 3000, TOL)
-    print(f"SU({Nc}) group dslash kernel:", {2: "VALU wave per site", 3: "FP32 matrix cores"}.get(
-        api.lib().qcuGroupsDslashVariant(), "?"), "; FP64 hops:", {2: "VALU wave per site", 3: "FP64 matrix cores"}.get(
+    print(f"SU({Nc}) group dslash kernel:", {2: "VALU wave per site", 3: "FP32 matrix cores", 5: "FP32 matrix cores, packed RHS"}.get(
+        api.lib().qcuGroupsDslashVariant(), "?"), "; FP64 hops:", {2: "VALU wave per site", 3: "FP64 matrix cores", 5: "FP64 matrix cores, packed RHS"}.get(
         api.lib().qcuGroupsDslashVariant64(), "?"), flush=True)
     xs = [np.stack([x.complex(srcs[k].shape[1]) for x in groups[k][0]]) for k in range(len(GROUPS))]
     api.finalizeQcu()

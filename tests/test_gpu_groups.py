@@ -30,8 +30,8 @@ def _run(L, Nc, kappa, sizes, seed, inner=1, tol=1e-10, user_prec=2):
         srcs.append(b)
         groups.append((xs, bs))
     infos = api.invert_groups(groups, 3000, tol)
-    print("group dslash kernel:", {2: "VALU wave per site", 3: "FP32 matrix cores"}.get(
-        api.lib().qcuGroupsDslashVariant(), "?"), "; FP64 hops:", {2: "VALU wave per site", 3: "FP64 matrix cores"}.get(
+    print("group dslash kernel:", {2: "VALU wave per site", 3: "FP32 matrix cores", 5: "FP32 matrix cores, packed RHS"}.get(
+        api.lib().qcuGroupsDslashVariant(), "?"), "; FP64 hops:", {2: "VALU wave per site", 3: "FP64 matrix cores", 5: "FP64 matrix cores, packed RHS"}.get(
         api.lib().qcuGroupsDslashVariant64(), "?"), flush=True)
     xs = [np.stack([x.complex(srcs[k].shape[1], user_prec) for x in groups[k][0]]) for k in range(len(sizes))]
     api.finalizeQcu()
@@ -61,6 +61,29 @@ def test_groups_match_oracle(Nc):
         assert infos[k]["converged"] and ok
         assert true_rel < 1.5e-10 and infos[k]["true_rel_residual"] < 1.5e-10
         assert err < 1e-8
+
+
+@pytest.mark.parametrize("Nc,inner,fp64", [(8, 5, 5), (4, 5, 3), (3, 3, 5), (8, 2, 3), (4, 3, 2)])
+def test_groups_each_kernel_matches_oracle(Nc, inner, fp64):
+    """The batched groups solve with its dslash kernels chosen explicitly
+    (qcuGroupsDslashTune): inner hops and FP64 hops on the VALU wave kernel (2),
+    the matrix cores per group (3) or the matrix cores over the groups' RHS
+    packed into shared 16-RHS chunks (5; groups of 3, 8, 17, 32 share 4 chunks),
+    every group to a FP64 true residual <= 1e-10 through the oracle operator."""
+    L, kappa, tol = (8, 8, 8, 8), 0.12, 1e-10
+    _case("groups forced", Nc, inner, fp64)
+    api.check(api.lib().qcuGroupsDslashTune(inner, fp64))
+    try:
+        gauge, srcs, xs, infos = _run(L, Nc, kappa, SIZES, 4300 + Nc)
+    finally:
+        api.check(api.lib().qcuGroupsDslashTune(0, 0))
+    for k, R in enumerate(SIZES):
+        bm = fields.to_mrhs(srcs[k], Nc)
+        r = bm - oracle.mat(fields.to_mrhs(xs[k], Nc), gauge, L, Nc, R, kappa)
+        true_rel = (oracle.norm(r, R) / oracle.norm(bm, R)).max()
+        print(f"Nc {Nc} inner {inner} fp64 {fp64} group R={R}: {infos[k]['iterations']} inner iterations, "
+              f"{infos[k]['outer_steps']} outer steps, true residual {true_rel:.2e}")
+        assert infos[k]["converged"] and true_rel < 1.5e-10
 
 
 def test_groups_single_group_equals_plain_solver():
