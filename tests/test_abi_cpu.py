@@ -105,3 +105,22 @@ def test_ctypes_prototypes_match_the_headers():
                 assert t == want, (name, p, t, want)
         checked += 1
     assert checked == len(decl) >= 50, (checked, len(decl))
+
+
+def test_launch_shape_setters_validate_without_the_gpu():
+    """qcuDslashTune and qcuGroupsDslashTune only record a process-wide choice:
+    out-of-range variants are rejected with kQcuErrorInvalidValue (2) before
+    any GPU work, valid ones accepted, and 0 restores the automatic choice."""
+    L = _lib.load()
+    for bad in ((4, 0), (0, 7), (1, 3), (-1, 0)):
+        assert L.qcuGroupsDslashTune(*bad) == 2, bad
+        assert L.qcuLastError() == 2
+        L.qcuClearError()
+    for ok in ((5, 5), (3, 2), (2, 3), (0, 0)):
+        assert L.qcuGroupsDslashTune(*ok) == 0, ok
+    for bad in ((9, 0, 0, 0), (3, -1, 0, 0), (2, 0, 0, -1), (4, 0, 0, -2)):
+        assert L.qcuDslashTune(*bad) == 2, bad
+        L.qcuClearError()
+    assert L.qcuDslashTune(4, 4, 8, 12) == 0
+    assert L.qcuDslashTune(0, 0, 0, 0) == 0
+    assert L.qcuLastError() == 0
