@@ -157,6 +157,7 @@ def train_network(w, args):
         api.loadQcuGauge(d_g, 2)
         if k == 0:
             api.getDslash(0, 1.0 / (2.0 * w["kappa"]) - 4.0)
+            lgnn.set_projected(args.lgnn_form == "projected")
             lgnn.set_weights(w1, w2)
             lgnn.clear_data()
         b = fields.gaussian_sources(Ltr, Nc, R, 7100 + k)
@@ -324,6 +325,9 @@ def main():
     ap.add_argument("--learned", default="half",
                     help="inner precisions also timed with the learned preconditioner (comma list; "
                          "'' = none); the network is trained first on 16^3x32 configurations")
+    ap.add_argument("--lgnn-form", default="projected", choices=["projected", "general"],
+                    help="network family: second-layer weights in the image of each hop's Wilson projector "
+                         "(hidden hop fields of two spins in the FP16/BF16 application), or general")
     ap.add_argument("--train-configs", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--train-power", type=int, default=3,
@@ -463,8 +467,9 @@ def main():
     timed = {}
     plan = [(m, None) for m in args.methods.split(",") if m]
     if trained is not None:
-        # the untrained start (first Neumann term) beside the trained network
+        # the untrained start (first Neumann term, in both families) beside the trained network
         plan += [(m, "neumann") for m in learned_methods] + [(m, "learned") for m in learned_methods]
+        lgnn.set_projected(args.lgnn_form == "projected")
     for m0, pc in plan:
         m = m0 + (f"+{pc}" if pc else "")
         if pc == "neumann":
@@ -561,7 +566,7 @@ def main():
                     for m, v in timed.items()},
         "dslash": dslash,
         "autotune": tuned,
-        "learned": ({"train_configs": args.train_configs, "train_lattice": [16, 16, 16, 32],
+        "learned": ({"form": args.lgnn_form, "train_configs": args.train_configs, "train_lattice": [16, 16, 16, 32],
                      "epochs": args.train_epochs, "loss": trained[2], "train_s": trained[3]}
                     if trained is not None else None),
         "learned_objective": (f"residual power {args.train_power}" if args.train_power else "pair loss")

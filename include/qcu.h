@@ -183,6 +183,16 @@ int qcuLgnnSetObjective(int power);
 /* One forward/backward pass of the residual-power loss on the loaded gauge
  * field for R FP32 odd vectors v: loss and gradients as for qcuLgnnLossGrad. */
 int qcuLgnnPowerLossGrad(const void *v, int power, double *loss, double *gw1, double *gw2);
+/* Projected network (on = 1; 0, the default, leaves the weights general):
+ * every second-layer weight of hop h = 1..8 is kept in the image of that
+ * hop's Wilson projector, W2[o][h] = P_h W2[o][h], P_h = (1 - gamma_mu) / 2
+ * forward and (1 + gamma_mu) / 2 backward -- the spin structure of the
+ * hopping term, which the first Neumann term already has.  The weights are
+ * projected now, whenever they are set and after every training step, and
+ * the FP16 / BF16 application stores two spins of each hidden hop field
+ * instead of four (half its traffic).  The mode persists across
+ * qcuLgnnSetWeights calls until switched off. */
+int qcuLgnnSetProjected(int on);
 
 /* ---- batched uneven right-hand-side groups (no reference counterpart) --
  * Solves M x = b for up to 8 groups of sources of different sizes on the

@@ -127,6 +127,7 @@ def load() -> ctypes.CDLL:
         "qcuLgnnTrain": ([i, d, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)], i),
         "qcuLgnnClearData": ([], i),
         "qcuLgnnSetObjective": ([i], i),
+        "qcuLgnnSetProjected": ([i], i),
         "qcuLgnnPowerLossGrad": ([vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_double)], i),
         "qcuTimerStart": ([], i),

@@ -523,7 +523,11 @@ __global__ __launch_bounds__(256) void k_dslash_mfma(uint32_t *__restrict__ out,
 // wavefronts of k_dslash_mfma hide their L1 misses; and the solver's Schur
 // operator would lose k_dslash_mfma's fused omega reduction.  So this form is
 // selected explicitly (qcuDslashTune / qcuDslashSetTuned / a tuning file), not
-// by the autotuner.
+// by the autotuner.  A workgroup over the segments of two adjacent rows (y hops
+// between them read from LDS: 4 input rows per output row instead of 5) was
+// slower, 262 us against 245 us for k_dslash_mfma: only two such 8-wavefront
+// workgroups fit a CU (83 VGPRs, 48 KB of LDS) against five of this one, and
+// capping the registers for a third spills (303 us).
 constexpr int kLdsSites = 4;
 
 using lds_void = __attribute__((address_space(3))) void;

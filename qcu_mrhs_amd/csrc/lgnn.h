@@ -33,9 +33,13 @@ hipError_t launch_gaussian(Cx<double> *out, long count, uint64_t key, hipStream_
 // the 32 x 32 (hop, spin) mixing on MFMA), then out(x) = M0 in(x) + sum_k
 // (T_k q_k)(x) on the odd sites (transport on MFMA).  q holds 8 fields of one
 // 32-bit word per complex; mcol = the collapsed stencil (M [32][32], M0 [4][4]).
+// projected: M[k][.] = P_k M[k][.] with P_k = (1 -+ gamma_mu) / 2 the Wilson projector of
+// hop k (second-layer weights in its image), so q_k has two independent spins and the
+// fields hold only spins 0 and 1 (half the field traffic); the gather rebuilds the others.
 constexpr size_t kLgnnFragBytes = 8 * 64 * 16;  // MFMA fragments of M stored after the q fields
 hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
-                            const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s);
+                            const int *walk, const Geom &g, int Nc, int R, int prec, bool projected,
+                            hipStream_t s);
 
 class LgnnPreconditioner : public Preconditioner {
 public:
@@ -54,6 +58,14 @@ public:
     // Weights: W1 [C][9][4][4], W2 [C][9][4][4] complex FP64 (host arrays).
     hipError_t set_weights(const double *w1, const double *w2, hipStream_t s);
     hipError_t get_weights(double *w1, double *w2, hipStream_t s);
+    // Projected network: every second-layer weight of hop h = 1..8 is kept in the image of
+    // that hop's Wilson projector, W2[o][h] = P_h W2[o][h] with P_h = (1 -+ gamma_mu) / 2
+    // (1 - gamma_mu forward, 1 + gamma_mu backward, the spin structure of the Wilson hopping
+    // term and of the first Neumann term): the weights are projected when set, when this mode
+    // is switched on and after every training step, and the FP16 / BF16 application stores
+    // half the hop fields.
+    hipError_t set_projected(bool on, hipStream_t s);
+    bool projected() const { return projected_; }
 
     // out = N in on odd-parity MRHS half vectors stored in precision `prec`
     hipError_t apply(void *out, const void *in, int prec, hipStream_t s) override;
@@ -80,6 +92,7 @@ private:
     hipError_t ensure_work32(hipStream_t s);
     hipError_t apply_mfma(void *out, const void *in, int prec, hipStream_t s);
     hipError_t collapse(hipStream_t s);
+    hipError_t project_w2(hipStream_t s);  // W2[o][h] = P_h W2[o][h], h = 1..8
     // stencil: collapsed stencil to apply (default the network's); hops: where the 8 hop
     // fields of the input go (default the work field u_)
     hipError_t forward(const void *in, int prec, void *out, int out_prec, hipStream_t s,
@@ -95,6 +108,7 @@ private:
     int Nc_, R_, C_;
     long n_;                        // complex entries per RHS of a half vector
     bool have_weights_ = false;
+    bool projected_ = false;
     const void *gauge_ = nullptr;   // FP64 gauge (borrowed)
     void *gauge32_ = nullptr;       // FP32 copy
     // parameters (FP64 complex) and Adam moments

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <type_traits>
 
+#include "dslash_common.h"
 #include "geom.h"
 #include "lgnn.h"
 
@@ -347,6 +348,33 @@ __global__ void k_adjoint_stencil(Cx<float> *adj, const Cx<float> *m) {
     }
 }
 
+// W2[o][h] = P_h W2[o][h] for hops h = 1 + k, k = 2 mu + (0 forward | 1 backward), with P_h the
+// orthogonal projector (1 -+ gamma_mu) / 2 onto the spinors whose spin m(r) is i^kr x spin r (r = 0, 1;
+// m and kr from the dslash tables of the hop without dagger): P[r][r] = P[m][m] = 1/2, P[m][r] = i^kr / 2,
+// P[r][m] = i^-kr / 2.  One thread per (o, h, column).
+__global__ void k_project_w2(C64 *w2, int C) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C * 8 * 4) return;
+    const int col = i & 3, k = (i >> 2) & 7, o = i >> 5;
+    const int mu = k >> 1;
+    const bool fwd = (k & 1) == 0;
+    const int sgn = fwd ? 2 : 0;
+    C64 *w = w2 + (size_t(o) * 9 + 1 + k) * 16;  // [4][4] row-major: w[s * 4 + col]
+    C64 v[4];
+    for (int s = 0; s < 4; ++s) v[s] = w[s * 4 + col];
+    for (int r = 0; r < 2; ++r) {
+        const int m = kMat2(mu, r), kr = (kReconK(mu, r) + sgn) & 3;
+        // i^kr as (re, im)
+        const double pr = kr == 0 ? 1.0 : kr == 2 ? -1.0 : 0.0, pi = kr == 1 ? 1.0 : kr == 3 ? -1.0 : 0.0;
+        const C64 a = v[r], b = v[m];
+        // (P v)_r = (a + i^-kr b) / 2, (P v)_m = (i^kr a + b) / 2
+        const C64 up = {0.5 * (a.re + pr * b.re + pi * b.im), 0.5 * (a.im + pr * b.im - pi * b.re)};
+        const C64 lo = {0.5 * (pr * a.re - pi * a.im + b.re), 0.5 * (pr * a.im + pi * a.re + b.im)};
+        w[r * 4 + col] = up;
+        w[m * 4 + col] = lo;
+    }
+}
+
 __global__ void k_adam(C64 *w, C64 *m, C64 *v, const C64 *g, int n, double lr, double b1, double b2, double eps,
                        double c1, double c2, double inv_batches) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -494,6 +522,20 @@ hipError_t LgnnPreconditioner::set_weights(const double *w1, const double *w2, h
     for (C64 *p : {g1_, g2_, m1_, m2_, v1_, v2_}) QCU_HIP_TRY(hipMemsetAsync(p, 0, wbytes, s));
     adam_t_ = 0;
     have_weights_ = true;
+    if (projected_) QCU_HIP_TRY(project_w2(s));
+    QCU_HIP_TRY(collapse(s));
+    return hipStreamSynchronize(s);
+}
+
+hipError_t LgnnPreconditioner::project_w2(hipStream_t s) {
+    hipLaunchKernelGGL(k_project_w2, dim3(div_up(C_ * 32, 256)), dim3(256), 0, s, w2_, C_);
+    return hipGetLastError();
+}
+
+hipError_t LgnnPreconditioner::set_projected(bool on, hipStream_t s) {
+    projected_ = on;
+    if (!on || !have_weights_) return hipSuccess;
+    QCU_HIP_TRY(project_w2(s));
     QCU_HIP_TRY(collapse(s));
     return hipStreamSynchronize(s);
 }
@@ -590,7 +632,7 @@ hipError_t LgnnPreconditioner::apply_mfma(void *out, const void *in, int prec, h
     if (!q16_) QCU_HIP_TRY(hipMalloc(&q16_, 8 * vec + kLgnnFragBytes));
     GeomTables tab;
     QCU_HIP_TRY(geom_tables(g_, g_.L[1], g_.L[2], tab, s));
-    return launch_lgnn_mfma(out, in, glo_[slot], q16_, mcol_, tab.walk, g_, Nc_, R_, prec, s);
+    return launch_lgnn_mfma(out, in, glo_[slot], q16_, mcol_, tab.walk, g_, Nc_, R_, prec, projected_, s);
 }
 
 hipError_t LgnnPreconditioner::zero_grad(hipStream_t s) {
@@ -771,6 +813,7 @@ hipError_t LgnnPreconditioner::adam_step(double lr, int batches, hipStream_t s) 
     hipLaunchKernelGGL(k_adam, dim3(div_up(n, 256)), dim3(256), 0, s, w1_, m1_, v1_, g1_, n, lr, b1, b2, eps, c1, c2, inv);
     hipLaunchKernelGGL(k_adam, dim3(div_up(n, 256)), dim3(256), 0, s, w2_, m2_, v2_, g2_, n, lr, b1, b2, eps, c1, c2, inv);
     QCU_HIP_TRY(hipGetLastError());
+    if (projected_) QCU_HIP_TRY(project_w2(s));  // back into the projected family after the step
     return collapse(s);
 }
 

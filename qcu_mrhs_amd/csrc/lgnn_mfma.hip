@@ -17,8 +17,17 @@
 // structure without spin projection: links as the A operand, the 4 spins of
 // q_k(nb_k) as B, one MFMA chain per spin; M0 psi(x) is added in the epilogue.
 // Both kernels read a walk-table row (site + 8 neighbours) per site.
+//
+// Projected form (PROJ, for a network whose second-layer weights of hop k lie
+// in the image of the Wilson projector P_k = (1 -+ gamma_mu) / 2 of that hop,
+// so that M[k][.] = P_k M[k][.] and every q_k satisfies the reconstruction
+// relation of the dslash: spin m(r) = i^kr(k) x spin r, r = 0, 1): the mix
+// computes and stores only spins 0 and 1 of each q_k (two MFMA row blocks
+// instead of four, fields of 2 spins), and the gather rebuilds spins 2 and 3
+// from them with the hop's phases before its MFMAs -- half the field traffic.
 #include <cstdint>
 
+#include "dslash_common.h"
 #include "lgnn.h"
 
 namespace qcu {
@@ -106,11 +115,12 @@ __device__ __forceinline__ void bstore_words(__amdgpu_buffer_rsrc_t r, u32x4 w, 
 // MFMA A-operand fragments of the real 2x2 embedding of M, in lane order:
 // frag[(rb * 2 + st) * 64 + lane] = 4 words (8 values) of rows (k, s, re|im) =
 // rb*16 + lane%16 and columns (k', t, re|im) = st*32 + (lane/16)*8 + 0..7.
-template <typename S>
+// PROJ: the rows (k, s < 2, re|im) only, 2 row blocks.
+template <typename S, bool PROJ>
 __global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restrict__ mcol) {
     using P = W<S>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 8 * 64) return;
+    if (i >= (PROJ ? 4 : 8) * 64) return;
     const int lane = i & 63, rbst = i >> 6, rb = rbst >> 1, st = rbst & 1;
     const int m = lane & 15, g = lane >> 4;
     u32x4 a;
@@ -120,7 +130,8 @@ __global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restri
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int row = rb * 16 + m, col = st * 32 + g * 8 + 2 * w + h;
-            const int ks = row >> 1, rho = row & 1, kt = col >> 1, sig = col & 1;
+            const int rk = row >> 1;  // (k, s) row of M: k 4 + s, or (k, s < 2) = k 2 + s projected
+            const int ks = PROJ ? (rk >> 1) * 4 + (rk & 1) : rk, rho = row & 1, kt = col >> 1, sig = col & 1;
             const Cx<float> c = mcol[ks * 32 + kt];
             v[h] = rho == sig ? c.re : (rho == 0 ? -c.im : c.im);
         }
@@ -131,11 +142,12 @@ __global__ void k_lgnn_frags(u32x4 *__restrict__ frag, const Cx<float> *__restri
 
 // q fields: 8 back-to-back fields [8][Vh][spin 4][R][NC] of 32-bit words (colour
 // innermost, unlike the solver's vectors: one lane's colours are one 8/12/16-byte
-// access in both kernels, and 16 lanes cover a contiguous run);
-// `fbytes` = bytes of one field (< 2^31, checked on the host).
+// access in both kernels, and 16 lanes cover a contiguous run), [8][Vh][spin 2][R][NC]
+// in the projected form; `fbytes` = bytes of one odd vector, 4 spins (< 2^31, checked
+// on the host).
 constexpr int kMixSites = 2;  // sites per wavefront of k_lgnn_mix
 
-template <typename S, int NC>
+template <typename S, int NC, bool PROJ>
 __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, const uint32_t *__restrict__ in,
                                                  const uint32_t *__restrict__ gauge, const u32x4 *__restrict__ frag,
                                                  const int *__restrict__ walk0, int Vh, int R, unsigned fbytes,
@@ -144,9 +156,10 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     // A operand: real embedding of M (8 fragments x 64 lanes x 16 bytes), staged once per
     // workgroup in LDS and read per row block, so that only one row block's accumulators
     // and A fragments are live at a time
-    __shared__ u32x4 sA[8 * 64];
+    constexpr int NRB = PROJ ? 2 : 4;  // row blocks of 16 rows (k, s, re|im)
+    __shared__ u32x4 sA[NRB * 2 * 64];
     sA[threadIdx.x] = frag[threadIdx.x];
-    sA[threadIdx.x + 256] = frag[threadIdx.x + 256];
+    if constexpr (!PROJ) sA[threadIdx.x + 256] = frag[threadIdx.x + 256];
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -215,12 +228,15 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
                 }
             }
         }
-        // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1.
-        // Fields 2 rb and 2 rb + 1 share one buffer resource; the lane's field is a 32-bit offset.
-        const unsigned lbase = unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
-                               unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
+        // lane (m, g) holds rows 4g + e of block rb: (k, s) = (2 rb + g/2, 2 (g & 1) + e/2), re|im = e & 1;
+        // fields 2 rb and 2 rb + 1 share one buffer resource, the lane's field is a 32-bit offset.
+        // Projected: (k, s) = (4 rb + g, e/2), fields 4 rb .. 4 rb + 3 (of 2 spins) behind one resource.
+        const unsigned qf = PROJ ? fbytes / 2u : fbytes;  // bytes of one q field
+        const unsigned lbase = PROJ ? unsigned(g) * qf + unsigned(site) * (site_bytes / 2u) + unsigned(r) * NC * 4u
+                                    : unsigned(g >> 1) * fbytes + unsigned(site) * site_bytes +
+                                          unsigned(2 * (g & 1)) * spin_bytes + unsigned(r) * NC * 4u;
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
+        for (int rb = 0; rb < NRB; ++rb) {
             const u32x4 a0 = sA[(rb * 2) * 64 + lane], a1 = sA[(rb * 2 + 1) * 64 + lane];
             f32x4 d[NC];
 #pragma unroll
@@ -229,7 +245,8 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
                 d[c] = P::mma(a1, bop[1][c], d[c]);
             }
             if (r < R) {
-                const auto r_q = rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
+                const auto r_q = PROJ ? rsrc(q + size_t(4 * rb) * (qf / 4u), 4u * qf)
+                                      : rsrc(q + size_t(2 * rb) * fwords, 2u * fbytes);
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
                     u32x4 w = {0u, 0u, 0u, 0u};
@@ -242,7 +259,15 @@ __global__ __launch_bounds__(256) void k_lgnn_mix(uint32_t *__restrict__ q, cons
     }
 }
 
-template <typename S, int NC, bool WIDE>
+// w times i^k on a packed (re, im) word of two 16-bit values: odd k swaps the halves, then the sign
+// bits of the negated halves flip (i: (-im, re), -1: (-re, -im), -i: (im, -re))
+__device__ __forceinline__ uint32_t times_i_pow(uint32_t w, int k) {
+    const uint32_t sw = __builtin_amdgcn_perm(w, w, (k & 1) ? 0x01000302u : 0x03020100u);
+    const uint32_t mask = k == 0 ? 0u : k == 1 ? 0x00008000u : k == 2 ? 0x80008000u : 0x80000000u;
+    return sw ^ mask;
+}
+
+template <typename S, int NC, bool WIDE, bool PROJ>
 __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out, const uint32_t *__restrict__ in,
                                                     const uint32_t *__restrict__ q, const uint32_t *__restrict__ gauge,
                                                     const Cx<float> *__restrict__ m0, const int *__restrict__ walk1,
@@ -276,7 +301,8 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
         const int lo = (g & 1) ? row[2 + 4 * st] : row[1 + 4 * st];
         const int hi = (g & 1) ? row[4 + 4 * st] : row[3 + 4 * st];
         const int nb = (g & 2) ? hi : lo;
-        const unsigned va = unsigned(nb) * site_bytes + unsigned(min(r, R - 1)) * NC * 4u;
+        const unsigned qf = PROJ ? fbytes / 2u : fbytes;  // bytes of one q field
+        const unsigned va = unsigned(nb) * (PROJ ? site_bytes / 2u : site_bytes) + unsigned(min(r, R - 1)) * NC * 4u;
         // field h of q at the neighbour: this lane's RHS, colours c' < NC of each spin (padding zero).
         // WIDE: one buffer resource spans the step's 4 fields, uniform over the wavefront, and the
         // lane group's field is a byte offset; otherwise a resource per field, which differs between
@@ -284,13 +310,27 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
         __amdgpu_buffer_rsrc_t r_q;
         unsigned vq = va;
         if constexpr (WIDE) {
-            r_q = rsrc(q + size_t(4 * st) * (fbytes / 4u), 4u * fbytes);
-            vq = va + unsigned(g) * fbytes;
+            r_q = rsrc(q + size_t(4 * st) * (qf / 4u), 4u * qf);
+            vq = va + unsigned(g) * qf;
         } else {
-            r_q = rsrc(q + size_t(h) * (fbytes / 4u), fbytes);
+            r_q = rsrc(q + size_t(h) * (qf / 4u), qf);
         }
 #pragma unroll
-        for (int sp = 0; sp < 4; ++sp) qw[st][sp] = bload_words<NC>(r_q, vq + unsigned(sp) * unsigned(R) * NC * 4u);
+        for (int sp = 0; sp < (PROJ ? 2 : 4); ++sp)
+            qw[st][sp] = bload_words<NC>(r_q, vq + unsigned(sp) * unsigned(R) * NC * 4u);
+        if constexpr (PROJ) {
+            // q_h lies in the image of the hop's projector: spin m(r) = i^kr x spin r, with m(0) = 3,
+            // m(1) = 2 for the x and y hops (step 0) and m(0) = 2, m(1) = 3 for z and t (step 1)
+            const HopInfo hi = hop_info(mu, fwd, false);
+            u32x4 lo0, lo1;
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                lo0[cc] = cc < NC ? times_i_pow(qw[st][0][cc], hi.kr0) : 0u;
+                lo1[cc] = cc < NC ? times_i_pow(qw[st][1][cc], hi.kr1) : 0u;
+            }
+            qw[st][st == 0 ? 3 : 2] = lo0;
+            qw[st][st == 0 ? 2 : 3] = lo1;
+        }
         // A word from link V[cb][c']: forward U(x)[cb][c'] (parity 1 at x), backward conj(U(nb)[c'][cb]) (parity 0)
         const unsigned gl = fwd ? (unsigned(mu) * 2u + 1u) * unsigned(Vh) + unsigned(site)
                                 : (unsigned(mu) * 2u) * unsigned(Vh) + unsigned(nb);
@@ -346,36 +386,37 @@ __global__ __launch_bounds__(256) void k_lgnn_gather(uint32_t *__restrict__ out,
     }
 }
 
-template <typename S, int NC>
+template <typename S, int NC, bool PROJ>
 hipError_t launch_nc(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol, const int *walk,
                      const Geom &g, int R, hipStream_t s) {
     const unsigned fbytes = unsigned(size_t(g.Vh) * 4 * NC * R * 4);
     const unsigned gbytes = unsigned(size_t(g.Vh) * 8 * NC * NC * 4);
     const dim3 grid(div_up(g.Vh, 4), div_up(R, 16)), block(256);
-    // the A fragments live after the 8 q fields
+    // the A fragments live after the 8 q fields (of 4 spins: the same place in the projected form)
     auto *frag = reinterpret_cast<u32x4 *>(static_cast<char *>(q) + 8 * size_t(fbytes));
-    hipLaunchKernelGGL((k_lgnn_frags<S>), dim3(2), dim3(256), 0, s, frag, mcol);
+    hipLaunchKernelGGL((k_lgnn_frags<S, PROJ>), dim3(PROJ ? 1 : 2), dim3(256), 0, s, frag, mcol);
     QCU_HIP_TRY(hipGetLastError());
     const dim3 grid_mix(div_up(g.Vh, 4 * kMixSites), div_up(R, 16));
-    hipLaunchKernelGGL((k_lgnn_mix<S, NC>), grid_mix, block, 0, s, static_cast<uint32_t *>(q),
+    hipLaunchKernelGGL((k_lgnn_mix<S, NC, PROJ>), grid_mix, block, 0, s, static_cast<uint32_t *>(q),
                        static_cast<const uint32_t *>(in), static_cast<const uint32_t *>(gauge), frag, walk, g.Vh, R,
                        fbytes, gbytes);
     QCU_HIP_TRY(hipGetLastError());
-    // four fields behind one buffer resource while 4 x fbytes stays below 2^31
-    auto gather = 4.0 * double(fbytes) < 2147483647.0 ? k_lgnn_gather<S, NC, true> : k_lgnn_gather<S, NC, false>;
+    // four fields behind one buffer resource while their bytes stay below 2^31
+    const double qf = PROJ ? 0.5 * double(fbytes) : double(fbytes);
+    auto gather = 4.0 * qf < 2147483647.0 ? k_lgnn_gather<S, NC, true, PROJ> : k_lgnn_gather<S, NC, false, PROJ>;
     hipLaunchKernelGGL(gather, grid, block, 0, s, static_cast<uint32_t *>(out), static_cast<const uint32_t *>(in),
                        static_cast<const uint32_t *>(q), static_cast<const uint32_t *>(gauge), mcol + 1024,
                        walk + size_t(g.Vh) * 9, g.Vh, R, fbytes, gbytes);
     return hipGetLastError();
 }
 
-template <typename S>
+template <typename S, bool PROJ>
 hipError_t launch_s(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol, const int *walk,
                     const Geom &g, int Nc, int R, hipStream_t s) {
     switch (Nc) {
-        case 2: return launch_nc<S, 2>(out, in, gauge, q, mcol, walk, g, R, s);
-        case 3: return launch_nc<S, 3>(out, in, gauge, q, mcol, walk, g, R, s);
-        case 4: return launch_nc<S, 4>(out, in, gauge, q, mcol, walk, g, R, s);
+        case 2: return launch_nc<S, 2, PROJ>(out, in, gauge, q, mcol, walk, g, R, s);
+        case 3: return launch_nc<S, 3, PROJ>(out, in, gauge, q, mcol, walk, g, R, s);
+        case 4: return launch_nc<S, 4, PROJ>(out, in, gauge, q, mcol, walk, g, R, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -383,13 +424,17 @@ hipError_t launch_s(void *out, const void *in, const void *gauge, void *q, const
 }  // namespace
 
 hipError_t launch_lgnn_mfma(void *out, const void *in, const void *gauge, void *q, const Cx<float> *mcol,
-                            const int *walk, const Geom &g, int Nc, int R, int prec, hipStream_t s) {
+                            const int *walk, const Geom &g, int Nc, int R, int prec, bool projected, hipStream_t s) {
     if (Nc < 2 || Nc > 4 || R < 1 || !out || !in || !gauge || !q || !mcol || !walk) return hipErrorInvalidValue;
     // two q fields share one buffer resource: 2 x (Vh 4 Nc R words) must stay below 2^31 bytes
     if (double(g.Vh) * 32.0 * Nc * R >= 2147483647.0 || double(g.Vh) * 32.0 * Nc * Nc >= 2147483647.0)
         return hipErrorInvalidValue;
-    if (prec == kHalf) return launch_s<__half>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
-    if (prec == kBF16) return launch_s<__hip_bfloat16>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
+    if (prec == kHalf)
+        return projected ? launch_s<__half, true>(out, in, gauge, q, mcol, walk, g, Nc, R, s)
+                         : launch_s<__half, false>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
+    if (prec == kBF16)
+        return projected ? launch_s<__hip_bfloat16, true>(out, in, gauge, q, mcol, walk, g, Nc, R, s)
+                         : launch_s<__hip_bfloat16, false>(out, in, gauge, q, mcol, walk, g, Nc, R, s);
     return hipErrorInvalidValue;
 }
 

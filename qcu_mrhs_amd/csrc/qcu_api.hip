@@ -79,6 +79,7 @@ struct Context {
     std::vector<void *> lgnn_gauges;  // FP64 copies, one per collected configuration
     std::vector<LgnnBatch> lgnn_batches;
     int lgnn_power = 0;  // training objective: 0 = (error, residual) pairs, K >= 1 = residual power K
+    bool lgnn_projected = false;  // second-layer weights kept in the image of each hop's Wilson projector
     hipStream_t stream = nullptr;
 
     void clear_lgnn_data() {
@@ -879,6 +880,7 @@ int qcuLgnnSetWeights(int channels, const double *w1, const double *w2) {
     }
     if (!c->precond || c->precond->channels() != channels)
         c->precond = std::make_unique<LgnnPreconditioner>(c->g, c->Nc, c->R, channels);
+    if (!hip_ok(c->precond->set_projected(c->lgnn_projected, c->stream), "lgnn projection")) return g_err;
     if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge")) return g_err;
     return hip_ok(c->precond->set_weights(w1, w2, c->stream), "lgnn weights") ? 0 : g_err;
 }
@@ -1068,6 +1070,19 @@ int qcuLgnnSetObjective(int power) {
         return g_err;
     }
     c->lgnn_power = power;
+    return 0;
+}
+
+int qcuLgnnSetProjected(int on) {
+    Context *c = ctx_or_error();
+    if (!c) return g_err;
+    if (on != 0 && on != 1) {
+        set_error(kQcuErrorInvalidValue, "qcuLgnnSetProjected: on must be 0 or 1");
+        return g_err;
+    }
+    c->lgnn_projected = on == 1;
+    if (c->precond && !hip_ok(c->precond->set_projected(c->lgnn_projected, c->stream), "lgnn projection"))
+        return g_err;
     return 0;
 }
 

@@ -128,6 +128,29 @@ def set_objective(power):
     api.check(api.lib().qcuLgnnSetObjective(int(power)))
 
 
+def projector(h):
+    """The Wilson projector of hop h = 1..8 (h = 1 + 2 mu forward, 2 + 2 mu
+    backward): (1 - gamma_mu) / 2 forward, (1 + gamma_mu) / 2 backward."""
+    mu, back = (h - 1) >> 1, (h - 1) & 1
+    return 0.5 * (np.eye(4) + (1 if back else -1) * GAMMA[mu])
+
+
+def project_w2(w2):
+    """Second-layer weights moved into the projected family: W2[o][h] =
+    P_h W2[o][h] for the hops h = 1..8 (hop 0, the identity, unchanged)."""
+    w2 = np.array(w2, dtype=np.complex128, copy=True)
+    for h in range(1, NHOP):
+        w2[:, h] = np.einsum("st,otu->osu", projector(h), w2[:, h])
+    return w2
+
+
+def set_projected(on):
+    """Projected network mode of the library (second-layer weights kept in
+    the image of each hop's Wilson projector; the FP16 / BF16 application then
+    stores half the hidden hop fields).  Persists across set_weights."""
+    api.check(api.lib().qcuLgnnSetProjected(1 if on else 0))
+
+
 def power_loss_grad(v, power, channels):
     """(loss, dL/dRe W1 + i dL/dIm W1, same for W2) of the residual-power loss
     sum_r ||(1 - A N)^power v_r / ||v_r||||^2 for one batch of FP32 odd-parity

@@ -126,22 +126,26 @@ def test_fp32_fp64_matrix_core_dslash_matches_oracle(L, Nc, R, prec):
             assert _rel(d_out.complex(inp.size, prec), xp - 0.13 * ref) < TOL[prec], (spw, "xpay")
 
 
-@pytest.mark.parametrize("L,Nc,R,ts", [((8, 4, 4, 6), 3, 17, 4), ((4, 6, 4, 8), 2, 3, 5), ((4, 4, 6, 4), 4, 5, 1),
-                                       ((12, 4, 4, 8), 3, 16, 3), ((36, 4, 4, 4), 3, 16, 2),
-                                       ((20, 4, 4, 6), 4, 32, 6), ((6, 4, 4, 4), 3, 12, 48),
-                                       ((2, 4, 2, 4), 3, 4, 3), ((10, 2, 4, 2), 2, 20, 1)])
-def test_lds_staged_dslash_matches_oracle(L, Nc, R, ts):
+@pytest.mark.parametrize("L,Nc,R,ts,by,bz", [((8, 4, 4, 6), 3, 17, 4, 0, 0), ((4, 6, 4, 8), 2, 3, 5, 0, 0),
+                                             ((4, 4, 6, 4), 4, 5, 1, 0, 0), ((12, 4, 4, 8), 3, 16, 3, 0, 0),
+                                             ((36, 4, 4, 4), 3, 16, 2, 0, 0), ((20, 4, 4, 6), 4, 32, 6, 0, 0),
+                                             ((6, 4, 4, 4), 3, 12, 48, 0, 0), ((2, 4, 2, 4), 3, 4, 3, 0, 0),
+                                             ((10, 2, 4, 2), 2, 20, 1, 0, 0), ((8, 8, 4, 4), 3, 16, 2, 4, 2),
+                                             ((4, 6, 6, 4), 3, 5, 3, 6, 3), ((6, 12, 2, 6), 4, 16, 48, 12, 2),
+                                             ((4, 6, 4, 4), 2, 3, 2, 3, 4)])
+def test_lds_staged_dslash_matches_oracle(L, Nc, R, ts, by, bz):
     """LDS-staged MFMA kernel (a workgroup per 4-site segment of an x-row,
     sweeping `ts` t steps with a ring of three haloed full rows and four
     spin-projected side rows): ragged last t runs, Ts = 1 and Ts beyond Lt,
     segments running past the row (Lx/2 = 18, 10, 5) and rows shorter than a
-    segment (Lx/2 = 1, 3), Nc 2/3/4, partial 16-RHS chunks, extents of 2,
-    both parities, dagger and the xpay form, FP16 and BF16, against the
-    oracle."""
-    print("case lds", L, Nc, R, ts, flush=True)
+    segment (Lx/2 = 1, 2, 3), Nc 2/3/4, partial 16-RHS chunks, extents of 2,
+    explicit (y, z) blocks of the workgroup order (the whole axis, odd
+    blocks), both parities, dagger and the xpay form, FP16 and BF16, against
+    the oracle."""
+    print("case lds", L, Nc, R, ts, by, bz, flush=True)
     gauge = fields.gauge_field(L, Nc, 900 + Nc + R, kind="haar")
     src = fields.gaussian_sources(L, Nc, R, 910 + R).reshape(R, 2, -1)
-    api.dslash_tune("lds", 0, 0, ts)
+    api.dslash_tune("lds", by, bz, ts)
     for prec in (H, B):
         d_g = api.DeviceArray.from_complex(gauge.reshape(-1), prec)
         for parity, dagger in ((0, 0), (1, 1), (1, 0), (0, 1)):

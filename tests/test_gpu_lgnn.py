@@ -95,6 +95,47 @@ def test_forward_matrix_core_path_shapes(Nc_, R_, prec):
     del d_g
 
 
+@pytest.mark.parametrize("Nc_,R_,prec", [(3, 4, 0), (3, 17, 0), (2, 16, 4), (4, 5, 0), (3, 32, 4)])
+def test_projected_network_matrix_core_path(Nc_, R_, prec):
+    """Projected network (second-layer weights in the image of each hop's
+    Wilson projector): the library projects the weights it is given as the
+    package's project_w2 does, and its FP16 / BF16 application -- hidden hop
+    fields of two spins, the other two rebuilt in the gather -- matches the
+    FP64 oracle network with the projected weights, on inputs rounded to the
+    storage precision; the same weights unprojected give a different network
+    (the projection is not a no-op here)."""
+    _case("lgnn projected", Nc_, R_, prec)
+    gauge = fields.gauge_field(L, Nc_, 271 + Nc_, kind="haar")
+    d_g = _setup(L, Nc_, R_, gauge, KAPPA)
+    Vh = fields.half_volume(L)
+    psi = _odd_field(272 + R_, (Vh, 4, Nc_, R_))
+    net = olgnn.Net.random(8, 273, scale=0.3)
+    w1, w2 = _abi_weights(net)
+    tol = 4e-3 if prec == 0 else 2.5e-2
+    lgnn.set_projected(True)
+    try:
+        lgnn.set_weights(w1, w2)
+        gw1, gw2 = lgnn.get_weights(8)
+        w2p = lgnn.project_w2(w2)
+        assert np.array_equal(gw1, w1) and np.abs(gw2 - w2p).max() < 1e-14
+        d_in = api.DeviceArray.from_complex(psi.reshape(-1), prec)
+        d_out = api.DeviceArray(api.storage_bytes(psi.size, prec))
+        lgnn.apply(d_out, d_in, prec)
+        got = d_out.complex(psi.size, prec)
+    finally:
+        lgnn.set_projected(False)
+    psi_s = api.from_storage(api.to_storage(psi.reshape(-1), prec), prec).reshape(psi.shape)
+    tr = olgnn.Transport(gauge, L, Nc_)
+    ref, _ = olgnn.Net(net.W1, w2p[None]).forward(tr, psi_s, L, Nc_, R_)
+    ref_full, _ = net.forward(tr, psi_s, L, Nc_, R_)
+    err = _rel(got, ref.reshape(-1))
+    print(f"Nc {Nc_} R {R_} precision {prec}: projected forward relative error {err:.2e} "
+          f"(unprojected network differs by {_rel(ref_full, ref):.2e})")
+    assert err < tol and _rel(ref_full, ref) > 10 * tol
+    api.finalizeQcu()
+    del d_g
+
+
 def test_neumann_weights_reproduce_schur_neumann_term():
     """Weights (1 -+ gamma_mu) built by the package (not the oracle): the GPU
     network equals psi + kappa^2 D_oe D_eo psi = 2 psi - A psi, with A the odd
@@ -181,14 +222,17 @@ def _solve(b, use_pc, inner=2, tol=1e-10, delta=0.0):
     return x, info
 
 
-def _train(Lt, Rt, eps, kappa, n_cfg, epochs=12, lr=2e-3, power=3):
+def _train(Lt, Rt, eps, kappa, n_cfg, epochs=12, lr=2e-3, power=3, projected=False):
     """Batches from plain FP64 solves of n_cfg seeded configurations (their
     gauge fields; 16 iterate snapshots + 16 Gaussian errors per RHS for the
     pair loss), training from the first Neumann term with the residual-power
-    loss on the GPU.  Returns (held-out gauge, held-out sources, device gauge,
-    Neumann start weights, trained weights, losses)."""
+    loss on the GPU (projected: in the projected family, which the library
+    keeps for the rest of the process until switched off).  Returns (held-out
+    gauge, held-out sources, device gauge, Neumann start weights, trained
+    weights, losses)."""
     gauges = [fields.gauge_field(Lt, Nc, 7000 + k, kind="smooth", eps=eps) for k in range(n_cfg)]
     d_g = _setup(Lt, Nc, Rt, gauges[0], kappa)
+    lgnn.set_projected(projected)
     w1, w2 = lgnn.neumann_weights(kappa, channels=8, noise=1e-3, seed=11)
     lgnn.set_weights(w1, w2)
     lgnn.clear_data()
@@ -267,6 +311,43 @@ def test_training_beats_the_untrained_start_on_held_out_config():
             assert pc["seconds"] < init["seconds"]
         else:
             assert max(pc["iterations"], init["iterations"]) < plain["iterations"]
+    api.finalizeQcu()
+    del d_g
+
+
+def test_projected_training_on_held_out_config():
+    """The 8^4 training of the test above in the projected family: every
+    training step leaves the second-layer weights in the image of the hops'
+    Wilson projectors; on the held-out configuration the trained network needs
+    no more FP64 iterations than its start (the first Neumann term, which is
+    in the family) and fewer than the solver without it, and the FP16 solve
+    with the half-traffic matrix-core application converges to an FP64 true
+    residual of 1e-10 through the oracle operator in fewer iterations than
+    without the network."""
+    Lt, Rt, kappa = (8, 8, 8, 8), 4, 0.135
+    _case("lgnn train projected", Lt)
+    try:
+        held, b, d_g, neumann, trained, losses = _train(Lt, Rt, 0.3, kappa, 4, projected=True)
+        print("training loss per epoch:", " ".join(f"{v:.3e}" for v in losses))
+        assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]
+        w2 = trained[1]
+        assert np.abs(lgnn.project_w2(w2) - w2).max() < 1e-12 * np.abs(w2).max()
+        it = {}
+        for inner in (2, 0):
+            _, plain = _solve(b, False, inner)
+            lgnn.set_weights(*neumann)
+            _, init = _solve(b, True, inner)
+            lgnn.set_weights(*trained)
+            x, pc = _solve(b, True, inner)
+            tr = _true_rel(x, b, held, Lt, Rt, kappa)
+            print(f"inner {inner}: iterations plain {plain['iterations']}, Neumann start {init['iterations']}, "
+                  f"trained (projected) {pc['iterations']}; trained true residual {tr:.2e}")
+            assert pc["converged"] and pc["preconditioned"] == 1 and pc["fallback"] == 0 and tr < 1.5e-10
+            it[inner] = (plain["iterations"], init["iterations"], pc["iterations"])
+        assert it[2][2] <= it[2][1] < it[2][0]
+        assert it[0][2] < it[0][0]
+    finally:
+        lgnn.set_projected(False)
     api.finalizeQcu()
     del d_g
 

@@ -152,3 +152,55 @@ def test_power_loss_gradients_match_finite_differences(K):
                       netm.power_loss_and_grad(tr, schur, v, K, L, Nc, R)[0]) / (2 * eps)
                 an = G[idx].real if dz == 1.0 else G[idx].imag
                 assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (which, idx, dz, fd, an)
+
+
+def _recon_table(h):
+    """Spin reconstruction of the projected matrix-core gather for hop
+    h = 1 + 2 mu (+1 backward): spin m[r] = i^kr[r] x spin r, r = 0, 1 -- the
+    dslash's reconstruction without dagger (the (1 + gamma) form, plus a
+    factor -1 forward)."""
+    mu, back = (h - 1) >> 1, (h - 1) & 1
+    m = (3, 2) if mu < 2 else (2, 3)
+    k_plus = {0: (3, 3), 1: (2, 0), 2: (3, 1), 3: (0, 0)}[mu]
+    return m, tuple((k + (0 if back else 2)) & 3 for k in k_plus)
+
+
+def test_projected_family_has_two_spins_per_hop_field():
+    """The package's P_h = (1 -+ gamma_mu) / 2 is the orthogonal projector
+    onto the image of hop h's spin factor in the first Neumann term (the
+    oracle's gamma matrices, pinned by the Neumann test above); its image is
+    exactly the spinors whose spins 2, 3 follow from spins 0, 1 by the
+    reconstruction the projected gather applies; second-layer weights moved
+    into the family keep the Neumann term, and every collapsed block M[h][h']
+    then maps into that image."""
+    from qcu_mrhs_amd import lgnn as plgnn
+    np.testing.assert_allclose(plgnn.GAMMA, dense.GAMMA, rtol=0, atol=0)
+    g = np.random.default_rng(41)
+    for h in range(1, 9):
+        mu, back = (h - 1) >> 1, (h - 1) & 1
+        P = plgnn.projector(h)
+        S = np.eye(4) + (1 if back else -1) * dense.GAMMA[mu]
+        np.testing.assert_allclose(P @ P, P, atol=1e-15)
+        np.testing.assert_allclose(P, P.conj().T, atol=1e-15)
+        np.testing.assert_allclose(P @ S, S, atol=1e-15)
+        assert abs(np.trace(P) - 2) < 1e-15
+        m, kr = _recon_table(h)
+        v = P @ (g.standard_normal((4, 6)) + 1j * g.standard_normal((4, 6)))
+        for r in (0, 1):
+            np.testing.assert_allclose(v[m[r]], 1j ** kr[r] * v[r], atol=1e-14)
+        # and a spinor obeying the reconstruction is in the image
+        u = np.zeros(4, dtype=np.complex128)
+        u[0], u[1] = 0.3 - 1.1j, 0.7 + 0.2j
+        u[m[0]], u[m[1]] = 1j ** kr[0] * u[0], 1j ** kr[1] * u[1]
+        np.testing.assert_allclose(P @ u, u, atol=1e-15)
+    net = lgnn.Net.random(8, 11, scale=0.3)
+    w2p = plgnn.project_w2(net.W2[0])
+    np.testing.assert_allclose(plgnn.project_w2(w2p), w2p, atol=1e-15)
+    np.testing.assert_array_equal(w2p[:, 0], net.W2[0][:, 0])
+    _, M = lgnn.Net(net.W1, w2p[None]).collapsed()
+    for k in range(8):
+        P = plgnn.projector(k + 1)
+        for kp in range(8):
+            np.testing.assert_allclose(P @ M[k, kp], M[k, kp], atol=1e-13)
+    neu = _neumann_net(0.13)
+    np.testing.assert_allclose(plgnn.project_w2(neu.W2[0]), neu.W2[0], atol=1e-15)
