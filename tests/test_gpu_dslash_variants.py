@@ -52,13 +52,13 @@ def test_variant_matches_oracle(variant, L, Nc, R):
             assert _rel(got, xp - 0.13 * ref) < TOL[prec], (prec, "xpay")
 
 
-@pytest.mark.parametrize("variant", ["wave", "mfma"])
-@pytest.mark.parametrize("by,bz,spw", [(1, 1, 1), (2, 4, 3), (4, 2, 5), (8, 8, 2), (2, 4, -1), (8, 8, -5)])
+@pytest.mark.parametrize("variant,by,bz,spw",
+                         [(v, *t) for v in ("wave", "mfma") for t in ((1, 1, 1), (2, 4, 3), (4, 2, 5), (8, 8, 2))]
+                         + [("mfma", 2, 4, -1), ("mfma", 8, 8, -5)])
 def test_traversal_tiles_and_sites_per_wave(variant, by, bz, spw):
-    """Every (y, z) tiling and sites-per-wave count covers each site exactly once."""
+    """Every (y, z) tiling and sites-per-wave count covers each site exactly
+    once (spw < 0: the persistent grid of the MFMA kernel)."""
     L, Nc, R = (6, 8, 8, 6), 3, 8
-    if spw < 0 and variant != "mfma":
-        pytest.skip("the persistent grid (spw < 0) exists for the MFMA kernel only")
     prec = H if variant == "mfma" else D
     gauge = fields.gauge_field(L, Nc, 7, kind="haar")
     inp = fields.to_mrhs(fields.gaussian_sources(L, Nc, R, 8).reshape(R, 2, -1)[:, 0], Nc)
