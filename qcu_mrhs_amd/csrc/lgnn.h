@@ -53,8 +53,10 @@ public:
     int channels() const { return C_; }
 
     // Gauge field the transports use (FP64 even/odd layout, device); a
-    // single-precision copy is kept.  Must be set before apply().
-    hipError_t set_gauge(const void *gauge64, hipStream_t s);
+    // single-precision copy is kept.  Must be set before apply().  gen >= 0 names
+    // the content (the caller's load count): the same pointer and generation as
+    // the last call keep the converted copies; gen < 0 always converts.
+    hipError_t set_gauge(const void *gauge64, hipStream_t s, long long gen = -1);
     // Weights: W1 [C][9][4][4], W2 [C][9][4][4] complex FP64 (host arrays).
     hipError_t set_weights(const double *w1, const double *w2, hipStream_t s);
     hipError_t get_weights(double *w1, double *w2, hipStream_t s);
@@ -110,6 +112,7 @@ private:
     bool have_weights_ = false;
     bool projected_ = false;
     const void *gauge_ = nullptr;   // FP64 gauge (borrowed)
+    long long gauge_gen_ = -1;      // generation of gauge_'s content the copies hold (-1: unknown)
     void *gauge32_ = nullptr;       // FP32 copy
     // parameters (FP64 complex) and Adam moments
     C64 *w1_ = nullptr, *w2_ = nullptr;          // [C][9][16]

@@ -506,12 +506,16 @@ hipError_t LgnnPreconditioner::ensure_fields(hipStream_t s) {
     return hipSuccess;
 }
 
-hipError_t LgnnPreconditioner::set_gauge(const void *gauge64, hipStream_t s) {
+hipError_t LgnnPreconditioner::set_gauge(const void *gauge64, hipStream_t s, long long gen) {
+    if (gen >= 0 && gauge64 == gauge_ && gen == gauge_gen_ && gauge32_) return hipSuccess;  // copies current
     gauge_ = gauge64;
+    gauge_gen_ = -1;
     glo_valid_[0] = glo_valid_[1] = false;
     const long count = 8L * g_.Vh * Nc_ * Nc_;
     if (!gauge32_) QCU_HIP_TRY(hipMalloc(&gauge32_, sizeof(C32) * count));
-    return launch_convert(gauge32_, kSingle, gauge64, kDouble, count, nullptr, 1, s);
+    QCU_HIP_TRY(launch_convert(gauge32_, kSingle, gauge64, kDouble, count, nullptr, 1, s));
+    gauge_gen_ = gen;
+    return hipSuccess;
 }
 
 hipError_t LgnnPreconditioner::set_weights(const double *w1, const double *w2, hipStream_t s) {

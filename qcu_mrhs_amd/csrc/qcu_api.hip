@@ -65,6 +65,7 @@ struct Context {
     void *cnorm = nullptr;           // one double: norm of the inverse (singularity check)
     CloverSet clover_set;
     void *gauge64 = nullptr;
+    long long gauge_gen = 0;  // loads into gauge64 so far (copies derived from it compare this)
     void *gauge_lo[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     void *bufA = nullptr, *bufB = nullptr;  // full-lattice MRHS vectors, FP64-sized
     std::vector<void *> q_in, q_out;
@@ -294,6 +295,7 @@ void loadQcuGauge(void *gauge, int floatPrecision) {
             p = nullptr;
         }
     c->clover_dirty = true;
+    ++c->gauge_gen;  // before the copy: a failed load leaves no derived copy marked current
     if (!hip_ok(launch_convert(c->gauge64, kDouble, gauge, floatPrecision, c->gauge_count(), nullptr, 1,
                                c->stream), "loadQcuGauge"))
         return;
@@ -400,7 +402,7 @@ int qcuInvertEx(int max_iteration, double tol, QcuSolveInfo *info) {
             return g_err;
         }
         pc = c->precond.get();
-        if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "preconditioner gauge")) {
+        if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "preconditioner gauge")) {
             c->q_in.clear();
             c->q_out.clear();
             return g_err;
@@ -881,7 +883,7 @@ int qcuLgnnSetWeights(int channels, const double *w1, const double *w2) {
     if (!c->precond || c->precond->channels() != channels)
         c->precond = std::make_unique<LgnnPreconditioner>(c->g, c->Nc, c->R, channels);
     if (!hip_ok(c->precond->set_projected(c->lgnn_projected, c->stream), "lgnn projection")) return g_err;
-    if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge")) return g_err;
+    if (!hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "lgnn gauge")) return g_err;
     return hip_ok(c->precond->set_weights(w1, w2, c->stream), "lgnn weights") ? 0 : g_err;
 }
 
@@ -902,7 +904,7 @@ int qcuLgnnApply(void *out, const void *in, int precision) {
         set_error(kQcuErrorNotInitialized, "qcuLgnnApply: no preconditioner or invalid precision");
         return g_err;
     }
-    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "lgnn gauge");
     ok = ok && hip_ok(c->precond->apply(out, in, precision, c->stream), "lgnn apply");
     ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn apply");
     return ok ? 0 : g_err;
@@ -915,7 +917,7 @@ int qcuLgnnLossGrad(const void *in, const void *target, double *loss, double *gw
         set_error(kQcuErrorNotInitialized, "no learned preconditioner");
         return g_err;
     }
-    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "lgnn gauge");
     ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn grad");
     ok = ok && hip_ok(c->precond->loss_grad(in, target, loss, c->stream), "lgnn grad");
     if (ok && gw1 && gw2) ok = hip_ok(c->precond->read_grad(gw1, gw2, c->stream), "lgnn grad");
@@ -1057,7 +1059,7 @@ int qcuLgnnTrain(int epochs, double learning_rate, unsigned seed, double *loss_p
         }
         if (loss_per_epoch) loss_per_epoch[ep] = total / (double(order.size()) * c->R);
     }
-    ok = ok && hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    ok = ok && hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "lgnn gauge");
     ok = ok && hip_ok(hipStreamSynchronize(c->stream), "lgnn train");
     return ok ? 0 : g_err;
 }
@@ -1093,7 +1095,7 @@ int qcuLgnnPowerLossGrad(const void *v, int power, double *loss, double *gw1, do
         set_error(kQcuErrorInvalidValue, "qcuLgnnPowerLossGrad: needs weights, getDslash and 1 <= power <= 8");
         return g_err;
     }
-    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream), "lgnn gauge");
+    bool ok = hip_ok(c->precond->set_gauge(c->gauge64, c->stream, c->gauge_gen), "lgnn gauge");
     ok = ok && hip_ok(c->precond->zero_grad(c->stream), "lgnn power");
     ok = ok && hip_ok(c->precond->power_loss_grad(v, power, c->kappa, loss, c->stream), "lgnn power");
     if (ok && gw1 && gw2) ok = hip_ok(c->precond->read_grad(gw1, gw2, c->stream), "lgnn power");

@@ -136,6 +136,37 @@ def test_projected_network_matrix_core_path(Nc_, R_, prec):
     del d_g
 
 
+def test_application_follows_a_reloaded_gauge_field():
+    """The network keeps its single- and 16-bit copies of the gauge field
+    between calls while the loaded field is unchanged, and a loadQcuGauge of
+    another field into the same context replaces them: FP32 and FP16
+    applications before and after the reload against the oracle on each
+    field (and the two fields give different results)."""
+    _case("lgnn gauge reload")
+    gA = fields.gauge_field(L, Nc, 301, kind="haar")
+    gB = fields.gauge_field(L, Nc, 302, kind="haar")
+    d_g = _setup(L, Nc, R, gA, KAPPA)
+    Vh = fields.half_volume(L)
+    psi = _odd_field(303, (Vh, 4, Nc, R))
+    net = olgnn.Net.random(8, 304, scale=0.3)
+    lgnn.set_weights(*_abi_weights(net))
+    for gauge in (gA, gA, gB, gB, gA):
+        d_g.upload(gauge.reshape(-1))
+        api.loadQcuGauge(d_g, 2)
+        for prec, tol in ((1, RTOL_FWD), (0, 4e-3)):
+            d_in = api.DeviceArray.from_complex(psi.reshape(-1), prec)
+            d_out = api.DeviceArray(api.storage_bytes(psi.size, prec))
+            for _ in range(2):  # the second call reuses the copies
+                lgnn.apply(d_out, d_in, prec)
+                got = d_out.complex(psi.size, prec)
+                psi_s = api.from_storage(api.to_storage(psi.reshape(-1), prec), prec).reshape(psi.shape)
+                ref, _ = net.forward(olgnn.Transport(gauge, L, Nc), psi_s, L, Nc, R)
+                other, _ = net.forward(olgnn.Transport(gB if gauge is gA else gA, L, Nc), psi_s, L, Nc, R)
+                assert _rel(got, ref.reshape(-1)) < tol and _rel(other, ref) > 100 * tol
+    api.finalizeQcu()
+    del d_g
+
+
 def test_neumann_weights_reproduce_schur_neumann_term():
     """Weights (1 -+ gamma_mu) built by the package (not the oracle): the GPU
     network equals psi + kappa^2 D_oe D_eo psi = 2 psi - A psi, with A the odd
