@@ -82,6 +82,40 @@ def time_dslash(L, Nc, R, d_src_even, d_gauge64, prec, reps=20):
     return ms
 
 
+def time_lgnn_apply(L, Nc, R, d_src_even, d_gauge64, kappa, reps=20):
+    """Average time (ms per call of qcuLgnnApply, HIP events on the library's stream) of one FP16 and
+    one BF16 application of the learned preconditioner at the workload, in the general and in the
+    projected family, with the first Neumann term (plus the training's starting noise) as weights;
+    the input is the sources' even half in the storage precision."""
+    from qcu_mrhs_amd import lgnn
+
+    n = fields.half_volume(L) * 4 * Nc * R
+    api.initGridSize(L, Nc, R, 2, 0)
+    api.loadQcuGauge(d_gauge64, 2)
+    api.getDslash(0, 1.0 / (2.0 * kappa) - 4.0)
+    w1, w2 = lgnn.neumann_weights(kappa, channels=8, noise=1e-3, seed=11)
+    out = {}
+    for name, p in (("half", 0), ("bf16", 4)):
+        d_in = api.DeviceArray(n * 4)
+        d_out = api.DeviceArray(n * 4)
+        api.check(api.lib().qcuCopyComplexVector(d_in.ptr, p, d_src_even.ptr, 2, n))
+        for form in ("general", "projected"):
+            lgnn.set_projected(form == "projected")
+            lgnn.set_weights(w1, w2)
+            for _ in range(3):
+                lgnn.apply(d_out, d_in, p)
+            api.synchronize()
+            api.check(api.lib().qcuTimerStart())
+            for _ in range(reps):
+                lgnn.apply(d_out, d_in, p)
+            out[f"{name}_{form}_ms"] = api.lib().qcuTimerStopMs() / reps
+            log(f"learned preconditioner {name}, {form} family: {out[f'{name}_{form}_ms'] * 1e3:.1f} us per call")
+        d_in.free()
+        d_out.free()
+    api.finalizeQcu()
+    return out
+
+
 def time_clover_apply(L, Nc, R, d_gauge64, csw, kappa, reps=20):
     """Average kernel time (ms) of the clover application of the FP16 inner solve: out = A_oo in +
     coef * y on FP16 half vectors with the FP32 blocks of one parity, out = y (the last pass of the
@@ -335,6 +369,8 @@ def main():
     ap.add_argument("--clover-csw", type=float, default=1.0,
                     help="also time the Wilson-clover solve (plain FP64 and FP16 inner) with this c_sw "
                          "(0 = skip)")
+    ap.add_argument("--lgnn-apply-only", action="store_true",
+                    help="time only the FP16/BF16 application of the learned preconditioner (both families)")
     ap.add_argument("--dslash-only", action="store_true",
                     help="time only the dslash (all storage precisions) and a plain copy kernel "
                          "of one half vector (counter calibration)")
@@ -382,6 +418,10 @@ def main():
     d_src = api.DeviceArray(Vh * 4 * Nc * R * 16)
     ptrs = (ctypes.c_void_p * R)(*[d.ptr for d in d_b])
     api.check(api.lib().qcuColorSpinorGather(d_src.ptr, 2, ptrs, 2, Vh, Nc, R))
+
+    if args.lgnn_apply_only:
+        print(json.dumps({"workload": w["name"], "lgnn_apply": time_lgnn_apply(L, Nc, R, d_src, d_gauge, kappa)}))
+        return
 
     api.dslash_tune(args.dslash_variant)
     tuned = {}
